@@ -1,0 +1,179 @@
+/*
+ * hipnufft -- C ABI of the MI355X (gfx950) NUFFT library.
+ *
+ * Every pointer argument named *_d is a device pointer (HBM) owned by the
+ * caller; every size is in elements.  Complex data is interleaved complex64
+ * (float re, float im).  All calls are asynchronous on the plan's stream and
+ * return 0 on success or a negative code; hipnufft_last_error() gives the
+ * message of the last failure on the calling thread.
+ *
+ * Conventions (identical to the reference's NDFT, nudft_numpy.py:38-53):
+ *   - image arrays are C-ordered, axis 0 slowest;
+ *   - sample coordinate d (radians, in [-pi, pi], float32) pairs with image
+ *     axis d;
+ *   - image index i along an axis of size N is the frequency k = i - N/2;
+ *   - type 2 ("op") computes c_m = sum_k f_k exp(iflag * i k.x_m) with
+ *     iflag = -1 by default; its adjoint is the type 1 with the opposite sign.
+ *
+ * Interfaces replaced (reference file:line):
+ *   cufinufft Plan(...) / RawCufinufftPlan._make_plan  operators/interfaces/cufinufft.py:83-92
+ *   plan._setpts                                        operators/interfaces/cufinufft.py:94-113
+ *   Plan.execute (type 1 / type 2)                      operators/interfaces/cufinufft.py:127-133
+ *   finufft Plan.execute / execute_adjoint              operators/interfaces/finufft.py:64-76
+ *   grad plan (type 2, isign=+1)                        operators/interfaces/cufinufft.py:1227-1237
+ *   spread/interp-only plan (Pipe)                      operators/interfaces/cufinufft.py:1311-1318
+ *   SENSE op / adj / data consistency chunk loops       operators/interfaces/cufinufft.py:478-500,
+ *                                                       :743-770, :1084-1121
+ *   _coil_combine_kernel (fused into the adjoint)       operators/gpu_utils.py:12-24
+ */
+#ifndef HIPNUFFT_H
+#define HIPNUFFT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct hipnufft_plan_s hipnufft_plan;
+
+/* Error codes. */
+#define HIPNUFFT_OK 0
+#define HIPNUFFT_ERR_ARG -1     /* invalid argument (maps to ValueError) */
+#define HIPNUFFT_ERR_HIP -2     /* HIP runtime failure */
+#define HIPNUFFT_ERR_NOPTS -3   /* execute before setpts */
+#define HIPNUFFT_ERR_ALLOC -4   /* device allocation failed */
+
+/* Tunables.  Zero / negative fields mean "choose automatically". */
+typedef struct hipnufft_opts {
+  double upsampfac;     /* oversampling factor sigma (default 2.0)          */
+  int spreadinterponly; /* 1: grid == image, no FFT, no deconvolution       */
+  int tile[3];          /* bin / tile edge per axis (powers of two)          */
+  int spread_group;     /* transforms accumulated together in one spread WG */
+  int interp_group;     /* transforms gathered together in one interp WG    */
+  int max_subproblem;   /* max points per interp work item (dense bins split) */
+  int deterministic;    /* reserved: 1 = fixed-order spreading               */
+} hipnufft_opts;
+
+/* Fill *opts with defaults.  Replaces the cufinufft Plan **kwargs
+ * (gpu_method, gpu_sort, upsampfac, gpu_spreadinterponly;
+ * cufinufft.py:38-46, :83-92). */
+int hipnufft_default_opts(hipnufft_opts* opts);
+
+/* Create a plan.  type is 1 or 2 (the plan executes both directions, like
+ * finufft's Plan.execute / execute_adjoint, finufft.py:64-76); nmodes[ndim]
+ * is the image shape in C order; iflag is the sign of the type-2 exponent
+ * (-1 for op, +1 for the reference's gradient plan, cufinufft.py:1227-1237);
+ * ntrans is the number of transforms the workspace holds (n_trans).
+ * stream is a hipStream_t (NULL = default stream).
+ * Replaces Plan(type, n_modes, n_trans, eps, ...) (cufinufft.py:83-92). */
+int hipnufft_plan_create(int type, int ndim, const int64_t* nmodes, int iflag,
+                         int ntrans, double eps, const hipnufft_opts* opts,
+                         void* stream, hipnufft_plan** out);
+
+/* Set the non-uniform points: M samples, coordinate arrays x_d[M] on the
+ * device (radians, float32), one per image axis (unused ones NULL).  The
+ * points are binned and stably sorted on the device; the plan keeps its own
+ * sorted copy (the caller's buffers may be freed afterwards, unlike
+ * cufinufft's borrowed pointers, cufinufft.py:105-113).
+ * Replaces plan._setpts (cufinufft.py:94-113) / Plan.setpts (finufft.py:55-62). */
+int hipnufft_setpts(hipnufft_plan* plan, int64_t M, const float* x0_d,
+                    const float* x1_d, const float* x2_d);
+
+/* Type 2 over ntrans transforms: in_d[ntrans][prod(nmodes)] ->
+ * out_d[ntrans][M].  Replaces Plan.execute for type 2 (cufinufft.py:131-133)
+ * and finufft Plan.execute (finufft.py:71-76). */
+int hipnufft_execute(hipnufft_plan* plan, const void* in_d, void* out_d);
+
+/* Adjoint (type 1, sign -iflag): in_d[ntrans][M] -> out_d[ntrans][nmodes].
+ * Replaces Plan.execute for type 1 (cufinufft.py:127-129) and finufft
+ * Plan.execute_adjoint (finufft.py:64-69). */
+int hipnufft_execute_adjoint(hipnufft_plan* plan, const void* in_d, void* out_d);
+
+/* SENSE / multi-coil forward over B batches and C coils:
+ *   ksp_d[b][c][M] = scale * F(smaps_d[c] * img_d[b])   if smaps_d != NULL
+ *   ksp_d[b][c][M] = scale * F(img_d[b][c])             if smaps_d == NULL
+ * Replaces MRICufiNUFFT._op_sense_device / _op_calibless_device and the
+ * normalisation (cufinufft.py:389-405, :478-500, :609-620). */
+int hipnufft_sense_op(hipnufft_plan* plan, const void* img_d,
+                      const void* smaps_d, void* ksp_d, int B, int C,
+                      float scale);
+
+/* SENSE / multi-coil adjoint:
+ *   img_d[b] = scale * sum_c conj(smaps_d[c]) * F^H(density_d * ksp_d[b][c])
+ *   (img_d[b][c] = scale * F^H(density_d * ksp_d[b][c]) if smaps_d == NULL)
+ * density_d may be NULL.  Replaces _adj_op_sense_device,
+ * _adj_op_calibless_device and _coil_combine_kernel (cufinufft.py:743-770,
+ * :883-902; gpu_utils.py:12-24). */
+int hipnufft_sense_adj(hipnufft_plan* plan, const void* ksp_d,
+                       const void* smaps_d, const float* density_d,
+                       void* img_d, int B, int C, float scale);
+
+/* Data consistency  grad = scale * A^H D (scale * A x - obs) with the same
+ * SENSE / calibrationless conventions (D = diag(density_d), may be NULL); the
+ * residual lives in a plan-owned ntrans*M buffer.  Replaces
+ * _dc_sense_device / _dc_calibless_device (cufinufft.py:1084-1121,
+ * :1153-1178). */
+int hipnufft_sense_dc(hipnufft_plan* plan, const void* img_d,
+                      const void* smaps_d, const void* obs_d,
+                      const float* density_d, void* grad_d, int B, int C,
+                      float scale);
+
+/* Trajectory gradient of a (SENSE) type-2 transform by the analytic kernel
+ * derivative: with y[b][c] = scale * T_iflag(S_c img[b]) (S_c = smaps_d[c],
+ * conjugated when flags & 1; calibrationless when smaps_d == NULL, as in
+ * hipnufft_sense_op),
+ *   gtraj_d[m*ndim + d] = Re sum_{b,c} conj(dy_d[b][c][m]) * D_{m,d} y[b][c][m]
+ * where D_{m,d} is the derivative with respect to coordinate d of sample m
+ * (radians).  One type-2 chain per coil and no extra NUFFTs, replacing
+ * the ndim extra transforms of _backward_op_samples / _backward_adj_samples
+ * (autodiff.py:22-41, :69-86). */
+int hipnufft_sense_grad_traj(hipnufft_plan* plan, const void* img_d,
+                             const void* smaps_d, const void* dy_d, int B,
+                             int C, float scale, int iflag, int flags,
+                             float* gtraj_d);
+
+/* FP64-accumulated complex dot over n complex64 entries:
+ *   out_d[0] + i out_d[1] = sum_j (conj ? conj(a_j) : a_j) * b_j
+ * (double precision output on the device), reduced in a fixed order so the
+ * result is bit-reproducible.  ws_d is an optional caller workspace of
+ * HIPNUFFT_DOT_WS doubles (NULL: the library allocates one per call, which a
+ * HIP graph capture cannot contain).  Replaces the CG inner products
+ * xp.dot / xp.linalg.norm (extras/optim.py:872-877). */
+#define HIPNUFFT_DOT_WS 2048
+int hipnufft_dot_c64(const void* a_d, const void* b_d, int64_t n, int conj,
+                     double* out_d, double* ws_d, void* stream);
+
+/* Sorted-point introspection for bit-exact tests: copies the permutation
+ * (sorted position -> original index, M int32), the bin offsets
+ * (nbins+1 int32) and the spread lists (entries int32 and offsets
+ * nbins+1 int32) into caller device buffers; any pointer may be NULL.
+ * sizes[0..2] receive M, nbins, number of list entries. */
+int hipnufft_get_sort(const hipnufft_plan* plan, int32_t* perm_d,
+                      int32_t* bin_start_d, int32_t* list_d,
+                      int32_t* list_start_d, int64_t* sizes);
+
+/* Plan introspection: nf[3] fine grid (C order, leading 1s for ndim < 3),
+ * kernel width, beta, tile[3], number of points. */
+int hipnufft_plan_info(const hipnufft_plan* plan, int64_t* nf, int* width,
+                       double* beta, int* tile, int64_t* M);
+
+/* Bytes of device workspace the plan holds (fine grids, sorted points,
+ * lists, tables).  Used for HBM-budget admission. */
+int hipnufft_workspace_bytes(const hipnufft_plan* plan, size_t* bytes);
+
+/* Change the stream of subsequent calls. */
+int hipnufft_set_stream(hipnufft_plan* plan, void* stream);
+
+/* Free the plan and its workspace. */
+int hipnufft_plan_destroy(hipnufft_plan* plan);
+
+/* Message of the last error on this thread ("" if none). */
+const char* hipnufft_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HIPNUFFT_H */

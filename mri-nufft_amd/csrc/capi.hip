@@ -1,0 +1,575 @@
+// C ABI of libhipnufft: plan life cycle, points, transforms, SENSE entry
+// points and FP64 reductions.  Declarations and the reference interfaces each
+// entry point replaces are in include/hipnufft.h.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "internal.h"
+
+namespace hn {
+
+thread_local std::string g_last_error;
+
+int set_error(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+namespace {
+
+// ----------------------------------------------------------------------------
+// host-side kernel parameters (published FINUFFT choices, see the oracle)
+// ----------------------------------------------------------------------------
+void kernel_params(double eps, double sigma, int* w, double* beta) {
+  int ww;
+  if (sigma == 2.0)
+    ww = (int)std::ceil(-std::log10(eps / 10.0));
+  else
+    ww = (int)std::ceil(-std::log(eps) / (M_PI * std::sqrt(1.0 - 1.0 / sigma)));
+  ww = std::min(std::max(ww, 2), 16);
+  double bw;
+  if (sigma == 2.0)
+    bw = ww == 2 ? 2.20 : ww == 3 ? 2.26 : ww == 4 ? 2.38 : 2.30;
+  else
+    bw = 0.97 * M_PI * (1.0 - 1.0 / (2.0 * sigma));
+  *w = ww;
+  *beta = bw * ww;
+}
+
+int64_t next_smooth_even(int64_t n) {
+  if (n < 2) n = 2;
+  if (n & 1) ++n;
+  for (;; n += 2) {
+    int64_t m = n;
+    for (int f : {2, 3, 5})
+      while (m % f == 0) m /= f;
+    if (m == 1) return n;
+  }
+}
+
+// Gauss-Legendre nodes / weights on [-1, 1] (Newton on P_n)
+void gauss_legendre(int n, std::vector<double>& x, std::vector<double>& wt) {
+  x.assign(n, 0.0);
+  wt.assign(n, 0.0);
+  for (int i = 0; i < n; ++i) {
+    double z = std::cos(M_PI * (i + 0.75) / (n + 0.5));
+    double dp = 1.0;
+    for (int it = 0; it < 100; ++it) {
+      double p0 = 1.0, p1 = z;
+      for (int k = 2; k <= n; ++k) {
+        const double p2 = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p0) / k;
+        p0 = p1;
+        p1 = p2;
+      }
+      dp = n * (z * p1 - p0) / (z * z - 1.0);
+      const double dz = p1 / dp;
+      z -= dz;
+      if (std::fabs(dz) < 1e-15) break;
+    }
+    x[i] = z;
+    wt[i] = 2.0 / ((1.0 - z * z) * dp * dp);
+  }
+}
+
+double es_kernel(double u, int w, double beta) {
+  const double z = 2.0 * u / w;
+  const double a = 1.0 - z * z;
+  return a >= 0.0 ? std::exp(beta * (std::sqrt(a) - 1.0)) : 0.0;
+}
+
+// 1 / phihat(2 pi k / nf) for k = i - M/2, i < M
+std::vector<float> deconv(int64_t M, int64_t nf, int w, double beta) {
+  std::vector<double> x, wt;
+  gauss_legendre(400, x, wt);
+  const double half = 0.5 * w;
+  std::vector<double> u(x.size()), f(x.size());
+  for (size_t q = 0; q < x.size(); ++q) {
+    u[q] = half * (x[q] + 1.0) / 2.0;
+    f[q] = es_kernel(u[q], w, beta) * wt[q] * half / 2.0;
+  }
+  std::vector<float> out(M);
+  for (int64_t i = 0; i < M; ++i) {
+    const double xi = 2.0 * M_PI * (double)(i - M / 2) / (double)nf;
+    double s = 0.0;
+    for (size_t q = 0; q < x.size(); ++q) s += std::cos(xi * u[q]) * f[q];
+    out[i] = (float)(1.0 / (2.0 * s));
+  }
+  return out;
+}
+
+// lines per FFT workgroup: n * L elements held 16 per thread, plus the
+// twiddle table, within 64 KiB of LDS
+int lines_per_wg(int64_t n) {
+  int L = 16;
+  while (L > 1 && (n * L > kMaxElemsFFT || (size_t)n * (L + 2) * sizeof(float2) > 65536)) L >>= 1;
+  return L;
+}
+
+int make_axis(int64_t n, FFTAxis* ax, size_t* bytes) {
+  ax->n = (int)n;
+  ax->L = lines_per_wg(n);
+  if ((size_t)n * (ax->L + 2) * sizeof(float2) > 65536 || n > kMaxElemsFFT)
+    return set_error(HIPNUFFT_ERR_ARG, "fine grid axis too long for the LDS FFT (max 2730)");
+  int64_t m = n;
+  ax->nrad = 0;
+  while (m % 8 == 0 && ax->nrad < kMaxRadices) { ax->rad[ax->nrad++] = 8; m /= 8; }
+  if (m % 4 == 0) { ax->rad[ax->nrad++] = 4; m /= 4; }
+  if (m % 2 == 0) { ax->rad[ax->nrad++] = 2; m /= 2; }
+  while (m % 3 == 0 && ax->nrad < kMaxRadices) { ax->rad[ax->nrad++] = 3; m /= 3; }
+  while (m % 5 == 0 && ax->nrad < kMaxRadices) { ax->rad[ax->nrad++] = 5; m /= 5; }
+  if (m != 1) return set_error(HIPNUFFT_ERR_ARG, "fine grid size is not 2,3,5-smooth");
+  std::vector<float2> tw(n);
+  for (int64_t t = 0; t < n; ++t) {
+    const double a = -2.0 * M_PI * (double)t / (double)n;
+    tw[t] = make_float2((float)std::cos(a), (float)std::sin(a));
+  }
+  HN_HIP(hipMalloc((void**)&ax->tw, sizeof(float2) * n));
+  HN_HIP(hipMemcpy(ax->tw, tw.data(), sizeof(float2) * n, hipMemcpyHostToDevice));
+  *bytes += sizeof(float2) * n;
+  return 0;
+}
+
+int default_tile(int ndim, int d) {
+  if (ndim == 1) return 512;
+  if (ndim == 2) return 32;
+  return d == 2 ? 16 : 8;
+}
+
+int floor_pow2(int64_t x) {
+  int p = 1;
+  while ((int64_t)p * 2 <= x) p *= 2;
+  return p;
+}
+
+void free_plan(hipnufft_plan_s* p) {
+  if (!p) return;
+  for (int d = 0; d < 3; ++d) {
+    (void)hipFree(p->dec[d]);
+    (void)hipFree(p->fft[d].tw);
+    (void)hipFree(p->pt_cell[d]);
+    (void)hipFree(p->pt_off[d]);
+  }
+  (void)hipFree(p->pt_idx);
+  (void)hipFree(p->bin_start);
+  (void)hipFree(p->sub_tile);
+  (void)hipFree(p->sub_begin);
+  (void)hipFree(p->list_pts);
+  (void)hipFree(p->list_start);
+  (void)hipFree(p->grid);
+  (void)hipFree(p->kwork);
+  delete p;
+}
+
+int need_pts(const hipnufft_plan_s* p) {
+  if (!p) return set_error(HIPNUFFT_ERR_ARG, "null plan");
+  if (!p->has_pts) return set_error(HIPNUFFT_ERR_NOPTS, "setpts has not been called");
+  return 0;
+}
+
+int64_t img_size(const hipnufft_plan_s* p) { return p->N[0] * p->N[1] * p->N[2]; }
+
+int ensure_kwork(hipnufft_plan_s* p) {
+  const int64_t need = (int64_t)p->ntrans * std::max<int64_t>(p->M, 1);
+  if (p->kwork && p->kwork_cap >= need) return 0;
+  if (p->kwork) {
+    HN_HIP(hipFree(p->kwork));
+    p->bytes -= sizeof(float2) * p->kwork_cap;
+  }
+  p->kwork = nullptr;
+  if (hipMalloc((void**)&p->kwork, sizeof(float2) * need) != hipSuccess)
+    return set_error(HIPNUFFT_ERR_ALLOC, "cannot allocate the residual workspace");
+  p->kwork_cap = need;
+  p->bytes += sizeof(float2) * need;
+  return 0;
+}
+
+// chunk size for B*C transforms: whole batches when they fit
+int chunk_of(const hipnufft_plan_s* p, int C, bool sense) {
+  int T = p->ntrans;
+  if (sense && T >= C) T = (T / C) * C;
+  return std::max(T, 1);
+}
+
+__global__ void k_zero_f32(float* x, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = 0.f;
+}
+
+// ----------------------------------------------------------------------------
+// FP64 dot: fixed-order two-stage reduction (bit-reproducible)
+// ----------------------------------------------------------------------------
+constexpr int kDotBlocks = HIPNUFFT_DOT_WS / 2;
+
+__global__ __launch_bounds__(kThreads) void k_dot_partial(const float2* a, const float2* b,
+                                                          int64_t n, int conj, double2* part) {
+  __shared__ double sr[kThreads], si[kThreads];
+  double re = 0.0, im = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kThreads) {
+    const float2 x = a[i], y = b[i];
+    const double xr = x.x, xi = conj ? -(double)x.y : (double)x.y;
+    re += xr * y.x - xi * y.y;
+    im += xr * y.y + xi * y.x;
+  }
+  sr[threadIdx.x] = re;
+  si[threadIdx.x] = im;
+  __syncthreads();
+  for (int s = kThreads / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      sr[threadIdx.x] += sr[threadIdx.x + s];
+      si[threadIdx.x] += si[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = make_double2(sr[0], si[0]);
+}
+
+__global__ __launch_bounds__(kThreads) void k_dot_final(const double2* part, int nb, double* out) {
+  __shared__ double sr[kThreads], si[kThreads];
+  double re = 0.0, im = 0.0;
+  for (int i = threadIdx.x; i < nb; i += kThreads) {
+    re += part[i].x;
+    im += part[i].y;
+  }
+  sr[threadIdx.x] = re;
+  si[threadIdx.x] = im;
+  __syncthreads();
+  for (int s = kThreads / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      sr[threadIdx.x] += sr[threadIdx.x + s];
+      si[threadIdx.x] += si[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = sr[0];
+    out[1] = si[0];
+  }
+}
+
+}  // namespace
+
+int dot_c64(const float2* a, const float2* b, int64_t n, int conj, double* out, double* ws,
+            hipStream_t s) {
+  const int nb = (int)std::max<int64_t>(1, std::min<int64_t>(kDotBlocks, (n + kThreads - 1) / kThreads));
+  double2* part = reinterpret_cast<double2*>(ws);
+  if (!part) HN_HIP(hipMallocAsync((void**)&part, sizeof(double2) * nb, s));
+  hipLaunchKernelGGL(k_dot_partial, dim3(nb), dim3(kThreads), 0, s, a, b, n, conj, part);
+  HN_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kThreads), 0, s, part, nb, out);
+  HN_LAUNCH_CHECK();
+  if (!ws) HN_HIP(hipFreeAsync(part, s));
+  return 0;
+}
+
+}  // namespace hn
+
+using namespace hn;
+
+extern "C" {
+
+const char* hipnufft_last_error(void) { return g_last_error.c_str(); }
+
+int hipnufft_default_opts(hipnufft_opts* o) {
+  if (!o) return set_error(HIPNUFFT_ERR_ARG, "null opts");
+  o->upsampfac = 2.0;
+  o->spreadinterponly = 0;
+  o->tile[0] = o->tile[1] = o->tile[2] = 0;
+  o->spread_group = 0;
+  o->interp_group = 0;
+  o->max_subproblem = 0;
+  o->deterministic = 0;
+  return 0;
+}
+
+int hipnufft_plan_create(int type, int ndim, const int64_t* nmodes, int iflag, int ntrans,
+                         double eps, const hipnufft_opts* opts, void* stream,
+                         hipnufft_plan** out) {
+  g_last_error.clear();
+  if (!out) return set_error(HIPNUFFT_ERR_ARG, "null output pointer");
+  *out = nullptr;
+  if (type != 1 && type != 2) return set_error(HIPNUFFT_ERR_ARG, "type must be 1 or 2");
+  if (ndim < 1 || ndim > 3) return set_error(HIPNUFFT_ERR_ARG, "ndim must be 1, 2 or 3");
+  if (!nmodes) return set_error(HIPNUFFT_ERR_ARG, "null nmodes");
+  if (ntrans < 1) return set_error(HIPNUFFT_ERR_ARG, "ntrans must be >= 1");
+  if (!(eps > 0.0)) return set_error(HIPNUFFT_ERR_ARG, "eps must be > 0");
+  if (iflag != 1 && iflag != -1) return set_error(HIPNUFFT_ERR_ARG, "iflag must be +1 or -1");
+  hipnufft_opts o;
+  hipnufft_default_opts(&o);
+  if (opts) o = *opts;
+  if (!(o.upsampfac > 1.0)) o.upsampfac = 2.0;
+
+  hipnufft_plan_s* p = new (std::nothrow) hipnufft_plan_s();
+  if (!p) return set_error(HIPNUFFT_ERR_ALLOC, "host allocation failed");
+  p->type = type;
+  p->ndim = ndim;
+  p->iflag = iflag;
+  p->sign2 = (type == 2) ? iflag : -iflag;
+  p->ntrans = ntrans;
+  p->eps = eps;
+  p->upsampfac = o.upsampfac;
+  p->sio = o.spreadinterponly ? 1 : 0;
+  p->stream = (hipStream_t)stream;
+  kernel_params(eps, o.upsampfac, &p->w, &p->beta);
+  const int off = 3 - ndim;
+  for (int d = 0; d < ndim; ++d) {
+    if (nmodes[d] < 1 || nmodes[d] > (1 << 20)) {
+      free_plan(p);
+      return set_error(HIPNUFFT_ERR_ARG, "invalid mode count");
+    }
+    p->N[off + d] = nmodes[d];
+    int64_t nf;
+    if (p->sio) {
+      nf = nmodes[d];
+      if (nf < p->w) {
+        free_plan(p);
+        return set_error(HIPNUFFT_ERR_ARG, "spread/interp-only grid smaller than the kernel width");
+      }
+    } else {
+      nf = (int64_t)(o.upsampfac * (double)nmodes[d]);
+      nf = next_smooth_even(std::max<int64_t>(nf, 2 * p->w));
+    }
+    p->nf[off + d] = nf;
+    int t = o.tile[d] > 0 ? o.tile[d] : default_tile(ndim, d);
+    t = floor_pow2(std::max(t, 1));
+    t = std::min(t, floor_pow2(nf));
+    p->tile[off + d] = t;
+    p->nt[off + d] = (int)((nf + t - 1) / t);
+  }
+  p->nbins = (int64_t)p->nt[0] * p->nt[1] * p->nt[2];
+  if (p->nbins >= ((int64_t)1 << 30)) {
+    free_plan(p);
+    return set_error(HIPNUFFT_ERR_ARG, "too many tiles");
+  }
+  p->spread_group = o.spread_group > 0 ? o.spread_group : (ndim == 3 ? 2 : 4);
+  p->interp_group = o.interp_group > 0 ? o.interp_group : (ndim == 3 ? 1 : 4);
+  p->max_sub = o.max_subproblem > 0 ? o.max_subproblem : 1024;
+
+  int rc = 0;
+  // deconvolution tables (padded axes hold a single 1)
+  for (int d = 0; d < 3 && !rc; ++d) {
+    std::vector<float> dv(p->N[d], 1.0f);
+    if (!p->sio && d >= off) dv = deconv(p->N[d], p->nf[d], p->w, p->beta);
+    if (hipMalloc((void**)&p->dec[d], sizeof(float) * dv.size()) != hipSuccess ||
+        hipMemcpy(p->dec[d], dv.data(), sizeof(float) * dv.size(), hipMemcpyHostToDevice) !=
+            hipSuccess)
+      rc = set_error(HIPNUFFT_ERR_ALLOC, "cannot allocate deconvolution tables");
+    p->bytes += sizeof(float) * dv.size();
+  }
+  if (!rc && !p->sio) {
+    for (int d = off; d < 3 && !rc; ++d) rc = make_axis(p->nf[d], &p->fft[d], &p->bytes);
+    if (!rc) {
+      const size_t g = sizeof(float2) * (size_t)grid_size(p) * (size_t)ntrans;
+      if (hipMalloc((void**)&p->grid, g) != hipSuccess)
+        rc = set_error(HIPNUFFT_ERR_ALLOC, "cannot allocate the fine-grid workspace");
+      p->bytes += g;
+    }
+  }
+  if (rc) {
+    free_plan(p);
+    return rc;
+  }
+  *out = p;
+  return 0;
+}
+
+int hipnufft_setpts(hipnufft_plan* p, int64_t M, const float* x0, const float* x1,
+                    const float* x2) {
+  g_last_error.clear();
+  if (!p) return set_error(HIPNUFFT_ERR_ARG, "null plan");
+  if (M < 0) return set_error(HIPNUFFT_ERR_ARG, "negative number of points");
+  const float* xd[3] = {x0, x1, x2};
+  int rc = build_points(p, M, xd);
+  if (rc) return rc;
+  return 0;
+}
+
+int hipnufft_sense_op(hipnufft_plan* p, const void* img, const void* smaps, void* ksp, int B,
+                      int C, float scale) {
+  int rc = need_pts(p);
+  if (rc) return rc;
+  if (B < 1 || C < 1) return set_error(HIPNUFFT_ERR_ARG, "B and C must be >= 1");
+  if (!img || !ksp) return set_error(HIPNUFFT_ERR_ARG, "null data pointer");
+  const float2* x = (const float2*)img;
+  const float2* s = (const float2*)smaps;
+  float2* y = (float2*)ksp;
+  const int TT = B * C;
+  if (p->sio) {
+    if (s) return set_error(HIPNUFFT_ERR_ARG, "spread/interp-only plans take no smaps");
+    return launch_interp(p, x, img_size(p), TT, y, nullptr, nullptr, scale, 0);
+  }
+  const int T = chunk_of(p, C, s != nullptr);
+  for (int t0 = 0; t0 < TT; t0 += T) {
+    const int Tc = std::min(T, TT - t0);
+    if ((rc = fwd_to_grid(p, x, s, C, t0, Tc, 1.0f, p->sign2))) return rc;
+    if ((rc = launch_interp(p, p->grid, grid_size(p), Tc, y + (int64_t)t0 * p->M, nullptr,
+                            nullptr, scale, 0)))
+      return rc;
+  }
+  return 0;
+}
+
+int hipnufft_sense_adj(hipnufft_plan* p, const void* ksp, const void* smaps, const float* density,
+                       void* img, int B, int C, float scale) {
+  int rc = need_pts(p);
+  if (rc) return rc;
+  if (B < 1 || C < 1) return set_error(HIPNUFFT_ERR_ARG, "B and C must be >= 1");
+  if (!img || !ksp) return set_error(HIPNUFFT_ERR_ARG, "null data pointer");
+  const float2* y = (const float2*)ksp;
+  const float2* s = (const float2*)smaps;
+  float2* x = (float2*)img;
+  const int TT = B * C;
+  if (p->sio) {
+    if (s) return set_error(HIPNUFFT_ERR_ARG, "spread/interp-only plans take no smaps");
+    if (scale != 1.0f) return set_error(HIPNUFFT_ERR_ARG, "spread/interp-only adjoint has no scale");
+    return launch_spread(p, y, density, TT, x, img_size(p));
+  }
+  const int T = chunk_of(p, C, s != nullptr);
+  for (int t0 = 0; t0 < TT; t0 += T) {
+    const int Tc = std::min(T, TT - t0);
+    if ((rc = launch_spread(p, y + (int64_t)t0 * p->M, density, Tc, p->grid, grid_size(p))))
+      return rc;
+    if ((rc = grid_to_img(p, x, s, C, t0, Tc, scale, -p->sign2))) return rc;
+  }
+  return 0;
+}
+
+int hipnufft_sense_dc(hipnufft_plan* p, const void* img, const void* smaps, const void* obs,
+                      const float* density, void* grad, int B, int C, float scale) {
+  int rc = need_pts(p);
+  if (rc) return rc;
+  if (B < 1 || C < 1) return set_error(HIPNUFFT_ERR_ARG, "B and C must be >= 1");
+  if (!img || !obs || !grad) return set_error(HIPNUFFT_ERR_ARG, "null data pointer");
+  if (p->sio) return set_error(HIPNUFFT_ERR_ARG, "data consistency needs a full NUFFT plan");
+  if ((rc = ensure_kwork(p))) return rc;
+  const float2* x = (const float2*)img;
+  const float2* s = (const float2*)smaps;
+  const float2* y = (const float2*)obs;
+  float2* g = (float2*)grad;
+  const int TT = B * C;
+  const int T = chunk_of(p, C, s != nullptr);
+  for (int t0 = 0; t0 < TT; t0 += T) {
+    const int Tc = std::min(T, TT - t0);
+    if ((rc = fwd_to_grid(p, x, s, C, t0, Tc, 1.0f, p->sign2))) return rc;
+    if ((rc = launch_interp(p, p->grid, grid_size(p), Tc, p->kwork, y + (int64_t)t0 * p->M, density,
+                            scale, 1)))
+      return rc;
+    if ((rc = launch_spread(p, p->kwork, nullptr, Tc, p->grid, grid_size(p)))) return rc;
+    if ((rc = grid_to_img(p, g, s, C, t0, Tc, scale, -p->sign2))) return rc;
+  }
+  return 0;
+}
+
+int hipnufft_execute(hipnufft_plan* p, const void* in, void* out) {
+  if (!p) return set_error(HIPNUFFT_ERR_ARG, "null plan");
+  if (p->type == 2) return hipnufft_sense_op(p, in, nullptr, out, p->ntrans, 1, 1.0f);
+  return hipnufft_sense_adj(p, in, nullptr, nullptr, out, p->ntrans, 1, 1.0f);
+}
+
+int hipnufft_execute_adjoint(hipnufft_plan* p, const void* in, void* out) {
+  if (!p) return set_error(HIPNUFFT_ERR_ARG, "null plan");
+  if (p->type == 2) return hipnufft_sense_adj(p, in, nullptr, nullptr, out, p->ntrans, 1, 1.0f);
+  return hipnufft_sense_op(p, in, nullptr, out, p->ntrans, 1, 1.0f);
+}
+
+int hipnufft_sense_grad_traj(hipnufft_plan* p, const void* img, const void* smaps, const void* dy,
+                             int B, int C, float scale, int iflag, int flags, float* gtraj) {
+  int rc = need_pts(p);
+  if (rc) return rc;
+  if (B < 1 || C < 1) return set_error(HIPNUFFT_ERR_ARG, "B and C must be >= 1");
+  if (!img || !dy || !gtraj) return set_error(HIPNUFFT_ERR_ARG, "null data pointer");
+  if (p->sio) return set_error(HIPNUFFT_ERR_ARG, "trajectory gradient needs a full NUFFT plan");
+  if (iflag != 1 && iflag != -1) return set_error(HIPNUFFT_ERR_ARG, "iflag must be +1 or -1");
+  const float2* x = (const float2*)img;
+  const float2* s = (const float2*)smaps;
+  const float2* d = (const float2*)dy;
+  const int64_t n = p->M * p->ndim;
+  if (n > 0) {
+    hipLaunchKernelGGL(k_zero_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, p->stream,
+                       gtraj, n);
+    HN_LAUNCH_CHECK();
+  }
+  const int TT = B * C;
+  const int T = chunk_of(p, C, s != nullptr);
+  for (int t0 = 0; t0 < TT; t0 += T) {
+    const int Tc = std::min(T, TT - t0);
+    if ((rc = fwd_to_grid(p, x, s, C, t0, Tc, 1.0f, iflag, flags & 1))) return rc;
+    if ((rc = launch_grad_traj(p, p->grid, grid_size(p), Tc, d + (int64_t)t0 * p->M, scale,
+                               gtraj)))
+      return rc;
+  }
+  return 0;
+}
+
+int hipnufft_dot_c64(const void* a, const void* b, int64_t n, int conj, double* out, double* ws,
+                     void* stream) {
+  if (!a || !b || !out) return set_error(HIPNUFFT_ERR_ARG, "null pointer");
+  if (n < 0) return set_error(HIPNUFFT_ERR_ARG, "negative length");
+  return dot_c64((const float2*)a, (const float2*)b, n, conj, out, ws, (hipStream_t)stream);
+}
+
+int hipnufft_get_sort(const hipnufft_plan* p, int32_t* perm, int32_t* bin_start, int32_t* list,
+                      int32_t* list_start, int64_t* sizes) {
+  int rc = need_pts(p);
+  if (rc) return rc;
+  hipStream_t s = p->stream;
+  if (perm && p->M)
+    HN_HIP(hipMemcpyAsync(perm, p->pt_idx, sizeof(int32_t) * p->M, hipMemcpyDeviceToDevice, s));
+  if (bin_start)
+    HN_HIP(hipMemcpyAsync(bin_start, p->bin_start, sizeof(int32_t) * (p->nbins + 1),
+                          hipMemcpyDeviceToDevice, s));
+  if (list && p->nlist)
+    HN_HIP(hipMemcpyAsync(list, p->list_pts, sizeof(int32_t) * p->nlist, hipMemcpyDeviceToDevice, s));
+  if (list_start)
+    HN_HIP(hipMemcpyAsync(list_start, p->list_start, sizeof(int32_t) * (p->nbins + 1),
+                          hipMemcpyDeviceToDevice, s));
+  if (sizes) {
+    sizes[0] = p->M;
+    sizes[1] = p->nbins;
+    sizes[2] = p->nlist;
+  }
+  return 0;
+}
+
+int hipnufft_plan_info(const hipnufft_plan* p, int64_t* nf, int* width, double* beta, int* tile,
+                       int64_t* M) {
+  if (!p) return set_error(HIPNUFFT_ERR_ARG, "null plan");
+  for (int d = 0; d < 3; ++d) {
+    if (nf) nf[d] = p->nf[d];
+    if (tile) tile[d] = p->tile[d];
+  }
+  if (width) *width = p->w;
+  if (beta) *beta = p->beta;
+  if (M) *M = p->M;
+  return 0;
+}
+
+int hipnufft_workspace_bytes(const hipnufft_plan* p, size_t* bytes) {
+  if (!p || !bytes) return set_error(HIPNUFFT_ERR_ARG, "null pointer");
+  size_t b = p->bytes;
+  b += p->M_cap * (sizeof(int32_t) + p->ndim * (sizeof(int32_t) + sizeof(float)));
+  b += (p->list_cap + 2 * p->sub_cap + 2 * (p->nbins + 1)) * sizeof(int32_t);
+  *bytes = b;
+  return 0;
+}
+
+int hipnufft_set_stream(hipnufft_plan* p, void* stream) {
+  if (!p) return set_error(HIPNUFFT_ERR_ARG, "null plan");
+  p->stream = (hipStream_t)stream;
+  return 0;
+}
+
+int hipnufft_plan_destroy(hipnufft_plan* p) {
+  if (!p) return 0;
+  (void)hipStreamSynchronize(p->stream);
+  free_plan(p);
+  return 0;
+}
+
+}  // extern "C"

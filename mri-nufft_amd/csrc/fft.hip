@@ -1,0 +1,512 @@
+// Batched Stockham FFTs on the oversampled grid, fused with deconvolution,
+// zero-padding / cropping and the SENSE coil expand / conjugate combine.
+//
+// Replaces the FFT, deapodisation, pad and crop stages inside cufinufft's
+// Plan.execute (operators/interfaces/cufinufft.py:127-133), the SENSE
+// broadcast multiply (cufinufft.py:497) and the coil-combine kernel
+// (operators/gpu_utils.py:12-24, operators/toeplitz.py:299-321).
+//
+// Layout: the fine grid of transform t is grid[t][nf0][nf1][nf2] (C order,
+// axis 2 contiguous; 2D / 1D grids use leading axes of size 1).  Image index
+// i along an axis of size M is frequency k = i - M/2 and lives at fine-grid
+// position (k mod nf).  Every pass works in place on whole lines held in LDS:
+// a workgroup owns L lines (L adjacent columns for the strided axes, so
+// global accesses are L*8-byte contiguous segments), loads only the positions
+// that are non-zero (pruned input), runs all Stockham passes in LDS and
+// stores only the positions later stages read (pruned output).
+
+#include <hip/hip_runtime.h>
+
+#include "internal.h"
+
+namespace hn {
+namespace {
+
+constexpr int kEPT = kMaxElemsFFT / kThreads;  // complex elements per thread (16)
+
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+// a * (S i)
+template <int S>
+__device__ __forceinline__ float2 mul_si(float2 a) {
+  return S > 0 ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
+
+// y_k = sum_j v_j exp(S 2 pi i j k / R)
+template <int S>
+__device__ __forceinline__ void dft2(float2* v) {
+  const float2 t = v[0];
+  v[0] = cadd(t, v[1]);
+  v[1] = csub(t, v[1]);
+}
+
+template <int S>
+__device__ __forceinline__ void dft4(float2& v0, float2& v1, float2& v2, float2& v3) {
+  const float2 a0 = cadd(v0, v2), a1 = csub(v0, v2), a2 = cadd(v1, v3);
+  const float2 a3 = mul_si<S>(csub(v1, v3));
+  v0 = cadd(a0, a2);
+  v2 = csub(a0, a2);
+  v1 = cadd(a1, a3);
+  v3 = csub(a1, a3);
+}
+
+template <int S>
+__device__ __forceinline__ void dft4(float2* v) {
+  dft4<S>(v[0], v[1], v[2], v[3]);
+}
+
+template <int S>
+__device__ __forceinline__ void dft8(float2* v) {
+  float2 e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+  float2 o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+  dft4<S>(e0, e1, e2, e3);
+  dft4<S>(o0, o1, o2, o3);
+  const float h = 0.70710678118654752f;
+  const float2 w1 = cscale(cadd(o1, mul_si<S>(o1)), h);   // W8^1 o1
+  const float2 w2 = mul_si<S>(o2);                        // W8^2 o2
+  const float2 w3 = cscale(csub(mul_si<S>(o3), o3), h);   // W8^3 o3
+  v[0] = cadd(e0, o0);
+  v[4] = csub(e0, o0);
+  v[1] = cadd(e1, w1);
+  v[5] = csub(e1, w1);
+  v[2] = cadd(e2, w2);
+  v[6] = csub(e2, w2);
+  v[3] = cadd(e3, w3);
+  v[7] = csub(e3, w3);
+}
+
+template <int S>
+__device__ __forceinline__ void dft3(float2* v) {
+  const float c = 0.86602540378443865f;
+  const float2 t = cadd(v[1], v[2]);
+  const float2 s = cscale(mul_si<S>(csub(v[1], v[2])), c);
+  const float2 m = make_float2(v[0].x - 0.5f * t.x, v[0].y - 0.5f * t.y);
+  v[0] = cadd(v[0], t);
+  v[1] = cadd(m, s);
+  v[2] = csub(m, s);
+}
+
+template <int S>
+__device__ __forceinline__ void dft5(float2* v) {
+  const float c1 = 0.30901699437494742f, c2 = -0.80901699437494742f;
+  const float s1 = 0.95105651629515357f, s2 = 0.58778525229247313f;
+  const float2 a1 = cadd(v[1], v[4]), b1 = csub(v[1], v[4]);
+  const float2 a2 = cadd(v[2], v[3]), b2 = csub(v[2], v[3]);
+  const float2 x0 = v[0];
+  const float2 p1 = make_float2(x0.x + c1 * a1.x + c2 * a2.x, x0.y + c1 * a1.y + c2 * a2.y);
+  const float2 p2 = make_float2(x0.x + c2 * a1.x + c1 * a2.x, x0.y + c2 * a1.y + c1 * a2.y);
+  const float2 q1 = mul_si<S>(make_float2(s1 * b1.x + s2 * b2.x, s1 * b1.y + s2 * b2.y));
+  const float2 q2 = mul_si<S>(make_float2(s2 * b1.x - s1 * b2.x, s2 * b1.y - s1 * b2.y));
+  v[0] = make_float2(x0.x + a1.x + a2.x, x0.y + a1.y + a2.y);
+  v[1] = cadd(p1, q1);
+  v[4] = csub(p1, q1);
+  v[2] = cadd(p2, q2);
+  v[3] = csub(p2, q2);
+}
+
+template <int R, int S>
+__device__ __forceinline__ void dftR(float2* v) {
+  if constexpr (R == 2) dft2<S>(v);
+  if constexpr (R == 3) dft3<S>(v);
+  if constexpr (R == 4) dft4<S>(v);
+  if constexpr (R == 5) dft5<S>(v);
+  if constexpr (R == 8) dft8<S>(v);
+}
+
+// One Stockham radix-R pass over L interleaved lines of length n held in LDS
+// at buf[pos * LP + line].  Every thread first reads all its butterflies into
+// registers, then (after a barrier) writes the results: a single LDS buffer.
+template <int R, int S>
+__device__ __forceinline__ void stockham_pass(float2* buf, int n, int lgL, int LP, int Ns,
+                                              const float2* tw) {
+  constexpr int MAXB = (kEPT + R - 1) / R;
+  const int L = 1 << lgL;
+  const int nr = n / R;
+  const int nbf = nr << lgL;
+  const int tstep = n / (Ns * R);
+  float2 v[MAXB][R];
+#pragma unroll
+  for (int i = 0; i < MAXB; ++i) {
+    const int b = threadIdx.x + i * kThreads;
+    if (b < nbf) {
+      const int l = b & (L - 1), j = b >> lgL;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[i][r] = buf[(j + r * nr) * LP + l];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < MAXB; ++i) {
+    const int b = threadIdx.x + i * kThreads;
+    if (b < nbf) {
+      const int l = b & (L - 1), j = b >> lgL;
+      const int q = j / Ns;
+      const int k = j - q * Ns;
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        float2 w = tw[r * k * tstep];
+        if (S > 0) w.y = -w.y;
+        v[i][r] = cmul(v[i][r], w);
+      }
+      dftR<R, S>(v[i]);
+      const int base = q * Ns * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[(base + r * Ns) * LP + l] = v[i][r];
+    }
+  }
+  __syncthreads();
+}
+
+// All passes of the FFT of the L lines in buf; twl is the twiddle table
+// staged in LDS by stage_twiddles (visible after the caller's barrier).
+template <int S>
+__device__ void fft_lines(float2* buf, const float2* twl, const FFTAxis& ax, int lgL, int LP) {
+  int Ns = 1;
+  for (int p = 0; p < ax.nrad; ++p) {
+    const int R = ax.rad[p];
+    switch (R) {
+      case 2: stockham_pass<2, S>(buf, ax.n, lgL, LP, Ns, twl); break;
+      case 3: stockham_pass<3, S>(buf, ax.n, lgL, LP, Ns, twl); break;
+      case 4: stockham_pass<4, S>(buf, ax.n, lgL, LP, Ns, twl); break;
+      case 5: stockham_pass<5, S>(buf, ax.n, lgL, LP, Ns, twl); break;
+      default: stockham_pass<8, S>(buf, ax.n, lgL, LP, Ns, twl); break;
+    }
+    Ns *= R;
+  }
+}
+
+// twiddles go after the n * LP data elements of the dynamic LDS buffer
+__device__ __forceinline__ float2* stage_twiddles(float2* buf, const FFTAxis& ax, int LP) {
+  float2* twl = buf + ax.n * LP;
+  for (int i = threadIdx.x; i < ax.n; i += kThreads) twl[i] = ax.tw[i];
+  return twl;
+}
+
+__device__ __forceinline__ int kept_pos(int i, int M, int nf) {
+  int p = i - M / 2;
+  return p < 0 ? p + nf : p;
+}
+
+// ---------------------------------------------------------------------------
+// fastest axis, fused with the image side
+// ---------------------------------------------------------------------------
+struct RowsArgs {
+  const float2* img;
+  float2* img_out;
+  const float2* smaps;
+  int C, t0, T;
+  int conj_smaps;  // forward: multiply by conj(smap) instead of smap
+  float2* grid;
+  int64_t gstride;
+  int M0, M1, M2;
+  int nf0, nf1, nf2;
+  const float* dec0;
+  const float* dec1;
+  const float* dec2;
+  float scale;
+  FFTAxis ax;
+  int lgL;
+};
+
+// image rows -> deconvolve * scale (* smap) -> pad -> FFT -> grid rows
+template <int S>
+__global__ __launch_bounds__(kThreads) void k_rows_fwd(RowsArgs a) {
+  extern __shared__ float2 buf[];
+  const int L = 1 << a.lgL, LP = L + 1;
+  const float2* twl = stage_twiddles(buf, a.ax, LP);
+  const int rows = a.M0 * a.M1;
+  const int r0 = blockIdx.x * L;
+  const int t = blockIdx.y;
+  const int tt = a.t0 + t;
+  const int64_t N = (int64_t)rows * a.M2;
+  const float2* src = a.img + (int64_t)(a.smaps ? tt / a.C : tt) * N;
+  const float2* sm = a.smaps ? a.smaps + (int64_t)(tt % a.C) * N : nullptr;
+  const int n = a.nf2;
+
+  for (int idx = threadIdx.x; idx < a.M2 * L; idx += kThreads) {
+    const int l = idx / a.M2, x = idx - l * a.M2;
+    const int r = r0 + l;
+    float2 v = make_float2(0.f, 0.f);
+    if (r < rows) {
+      const int i0 = r / a.M1, i1 = r - i0 * a.M1;
+      const int64_t off = (int64_t)r * a.M2 + x;
+      v = src[off];
+      if (sm) {
+        float2 s = sm[off];
+        if (a.conj_smaps) s.y = -s.y;
+        v = cmul(v, s);
+      }
+      v = cscale(v, a.dec0[i0] * a.dec1[i1] * a.dec2[x] * a.scale);
+    }
+    buf[kept_pos(x, a.M2, n) * LP + l] = v;
+  }
+  const int gap0 = a.M2 - a.M2 / 2, gap = n - a.M2;
+  for (int idx = threadIdx.x; idx < gap * L; idx += kThreads) {
+    const int l = idx & (L - 1), e = idx >> a.lgL;
+    buf[(gap0 + e) * LP + l] = make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  fft_lines<S>(buf, twl, a.ax, a.lgL, LP);
+  float2* g = a.grid + (int64_t)t * a.gstride;
+  for (int idx = threadIdx.x; idx < n * L; idx += kThreads) {
+    const int l = idx / n, x = idx - l * n;
+    const int r = r0 + l;
+    if (r < rows) {
+      const int i0 = r / a.M1, i1 = r - i0 * a.M1;
+      const int64_t grow = ((int64_t)kept_pos(i0, a.M0, a.nf0) * a.nf1 + kept_pos(i1, a.M1, a.nf1)) * n;
+      g[grow + x] = buf[x * LP + l];
+    }
+  }
+}
+
+// grid rows -> FFT -> crop -> deconvolve * scale (* conj smap, summed over the
+// chunk's coils of one batch) -> image rows
+template <int S>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) void k_rows_adj(
+    RowsArgs a) {
+  extern __shared__ float2 buf[];
+  const int L = 1 << a.lgL, LP = L + 1;
+  const float2* twl = stage_twiddles(buf, a.ax, LP);
+  const int rows = a.M0 * a.M1;
+  const int r0 = blockIdx.x * L;
+  const int n = a.nf2;
+  const int64_t N = (int64_t)rows * a.M2;
+  int tb, te, out_img;
+  if (a.smaps) {  // one output image per batch present in the chunk
+    const int b = a.t0 / a.C + blockIdx.y;
+    out_img = b;
+    tb = max(a.t0, b * a.C);
+    te = min(a.t0 + a.T, (b + 1) * a.C);
+  } else {
+    out_img = a.t0 + blockIdx.y;
+    tb = out_img;
+    te = out_img + 1;
+  }
+  const bool add_existing = a.smaps && (tb % a.C != 0);
+  const int nout = a.M2 * L;
+  float2 acc[kEPT];
+#pragma unroll
+  for (int i = 0; i < kEPT; ++i) acc[i] = make_float2(0.f, 0.f);
+
+  for (int tt = tb; tt < te; ++tt) {
+    const float2* g = a.grid + (int64_t)(tt - a.t0) * a.gstride;
+    for (int idx = threadIdx.x; idx < n * L; idx += kThreads) {
+      const int l = idx / n, x = idx - l * n;
+      const int r = r0 + l;
+      float2 v = make_float2(0.f, 0.f);
+      if (r < rows) {
+        const int i0 = r / a.M1, i1 = r - i0 * a.M1;
+        const int64_t grow = ((int64_t)kept_pos(i0, a.M0, a.nf0) * a.nf1 + kept_pos(i1, a.M1, a.nf1)) * n;
+        v = g[grow + x];
+      }
+      buf[x * LP + l] = v;
+    }
+    __syncthreads();
+    fft_lines<S>(buf, twl, a.ax, a.lgL, LP);
+    const float2* sm = a.smaps ? a.smaps + (int64_t)(tt % a.C) * N : nullptr;
+#pragma unroll
+    for (int i = 0; i < kEPT; ++i) {
+      const int idx = threadIdx.x + i * kThreads;
+      if (idx < nout) {
+        const int l = idx / a.M2, x = idx - l * a.M2;
+        const int r = r0 + l;
+        if (r < rows) {
+          float2 v = buf[kept_pos(x, a.M2, n) * LP + l];
+          if (sm) {
+            float2 s = sm[(int64_t)r * a.M2 + x];
+            s.y = -s.y;
+            v = cmul(v, s);
+          }
+          acc[i] = cadd(acc[i], v);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  float2* dst = a.img_out + (int64_t)out_img * N;
+#pragma unroll
+  for (int i = 0; i < kEPT; ++i) {
+    const int idx = threadIdx.x + i * kThreads;
+    if (idx < nout) {
+      const int l = idx / a.M2, x = idx - l * a.M2;
+      const int r = r0 + l;
+      if (r < rows) {
+        const int i0 = r / a.M1, i1 = r - i0 * a.M1;
+        float2 v = cscale(acc[i], a.dec0[i0] * a.dec1[i1] * a.dec2[x] * a.scale);
+        const int64_t o = (int64_t)r * a.M2 + x;
+        if (add_existing) v = cadd(v, dst[o]);
+        dst[o] = v;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// strided axes (0 and 1), in place
+// ---------------------------------------------------------------------------
+struct StridedArgs {
+  float2* data;
+  int64_t tstride;
+  int n;
+  int64_t lstride;
+  int nouter, outer_M, outer_nf;  // outer_M > 0: outer index i maps to kept_pos
+  int64_t ostride;
+  int nx;
+  int in_M, out_M;  // 0: full line, > 0: kept positions only
+  FFTAxis ax;
+  int lgL;
+};
+
+template <int S>
+__global__ __launch_bounds__(kThreads) void k_fft_strided(StridedArgs a) {
+  extern __shared__ float2 buf[];
+  const int L = 1 << a.lgL, LP = L + 1;
+  const float2* twl = stage_twiddles(buf, a.ax, LP);
+  const int x0 = blockIdx.x * L;
+  const int oi = blockIdx.y;
+  const int t = blockIdx.z;
+  const int opos = a.outer_M ? kept_pos(oi, a.outer_M, a.outer_nf) : oi;
+  float2* base = a.data + (int64_t)t * a.tstride + (int64_t)opos * a.ostride + x0;
+  const int n = a.n;
+  const int nin = a.in_M ? a.in_M : n;
+  for (int idx = threadIdx.x; idx < nin * L; idx += kThreads) {
+    const int l = idx & (L - 1), e = idx >> a.lgL;
+    const int pos = a.in_M ? kept_pos(e, a.in_M, n) : e;
+    buf[pos * LP + l] = (x0 + l < a.nx) ? base[(int64_t)pos * a.lstride + l] : make_float2(0.f, 0.f);
+  }
+  if (a.in_M) {
+    const int gap0 = a.in_M - a.in_M / 2, gap = n - a.in_M;
+    for (int idx = threadIdx.x; idx < gap * L; idx += kThreads) {
+      const int l = idx & (L - 1), e = idx >> a.lgL;
+      buf[(gap0 + e) * LP + l] = make_float2(0.f, 0.f);
+    }
+  }
+  __syncthreads();
+  fft_lines<S>(buf, twl, a.ax, a.lgL, LP);
+  const int nout = a.out_M ? a.out_M : n;
+  for (int idx = threadIdx.x; idx < nout * L; idx += kThreads) {
+    const int l = idx & (L - 1), e = idx >> a.lgL;
+    const int pos = a.out_M ? kept_pos(e, a.out_M, n) : e;
+    if (x0 + l < a.nx) base[(int64_t)pos * a.lstride + l] = buf[pos * LP + l];
+  }
+}
+
+int lg2(int x) {
+  int l = 0;
+  while ((1 << l) < x) ++l;
+  return l;
+}
+
+size_t fft_lds(const FFTAxis& ax) { return sizeof(float2) * (size_t)ax.n * (ax.L + 2); }
+
+int strided(const hipnufft_plan_s* p, int axis, int T, int in_M, int out_M, int outer_M,
+            int sign) {
+  const FFTAxis& ax = p->fft[axis];
+  StridedArgs a{};
+  a.data = p->grid;
+  a.tstride = grid_size(p);
+  a.n = (int)p->nf[axis];
+  a.nx = (int)p->nf[2];
+  a.in_M = in_M;
+  a.out_M = out_M;
+  a.ax = ax;
+  a.lgL = lg2(ax.L);
+  if (axis == 1) {
+    a.lstride = p->nf[2];
+    a.ostride = p->nf[1] * p->nf[2];
+    a.outer_nf = (int)p->nf[0];
+  } else {
+    a.lstride = p->nf[1] * p->nf[2];
+    a.ostride = p->nf[2];
+    a.outer_nf = (int)p->nf[1];
+  }
+  a.outer_M = outer_M;
+  a.nouter = outer_M ? outer_M : a.outer_nf;
+  const dim3 grd((unsigned)((a.nx + ax.L - 1) / ax.L), (unsigned)a.nouter, (unsigned)T);
+  if (sign < 0)
+    hipLaunchKernelGGL(k_fft_strided<-1>, grd, dim3(kThreads), fft_lds(ax), p->stream, a);
+  else
+    hipLaunchKernelGGL(k_fft_strided<1>, grd, dim3(kThreads), fft_lds(ax), p->stream, a);
+  HN_LAUNCH_CHECK();
+  return 0;
+}
+
+RowsArgs rows_args(const hipnufft_plan_s* p) {
+  RowsArgs a{};
+  a.grid = p->grid;
+  a.gstride = grid_size(p);
+  a.M0 = (int)p->N[0];
+  a.M1 = (int)p->N[1];
+  a.M2 = (int)p->N[2];
+  a.nf0 = (int)p->nf[0];
+  a.nf1 = (int)p->nf[1];
+  a.nf2 = (int)p->nf[2];
+  a.dec0 = p->dec[0];
+  a.dec1 = p->dec[1];
+  a.dec2 = p->dec[2];
+  a.ax = p->fft[2];
+  a.lgL = lg2(p->fft[2].L);
+  return a;
+}
+
+}  // namespace
+
+int64_t grid_size(const hipnufft_plan_s* p) { return p->nf[0] * p->nf[1] * p->nf[2]; }
+
+int fwd_to_grid(const hipnufft_plan_s* p, const float2* img, const float2* smaps, int C, int t0,
+                int T, float scale, int sign, int conj_smaps) {
+  if (T <= 0) return 0;
+  RowsArgs a = rows_args(p);
+  a.conj_smaps = conj_smaps;
+  a.img = img;
+  a.smaps = smaps;
+  a.C = C;
+  a.t0 = t0;
+  a.T = T;
+  a.scale = scale;
+  const int rows = a.M0 * a.M1;
+  const int L = p->fft[2].L;
+  const dim3 grd((unsigned)((rows + L - 1) / L), (unsigned)T);
+  const size_t lds = fft_lds(p->fft[2]);
+  if (sign < 0)
+    hipLaunchKernelGGL(k_rows_fwd<-1>, grd, dim3(kThreads), lds, p->stream, a);
+  else
+    hipLaunchKernelGGL(k_rows_fwd<1>, grd, dim3(kThreads), lds, p->stream, a);
+  HN_LAUNCH_CHECK();
+  int rc;
+  if (p->ndim >= 2 && (rc = strided(p, 1, T, (int)p->N[1], 0, (int)p->N[0], sign))) return rc;
+  if (p->ndim == 3 && (rc = strided(p, 0, T, (int)p->N[0], 0, 0, sign))) return rc;
+  return 0;
+}
+
+int grid_to_img(const hipnufft_plan_s* p, float2* img, const float2* smaps, int C, int t0, int T,
+                float scale, int sign) {
+  if (T <= 0) return 0;
+  int rc;
+  if (p->ndim == 3 && (rc = strided(p, 0, T, 0, (int)p->N[0], 0, sign))) return rc;
+  if (p->ndim >= 2 && (rc = strided(p, 1, T, 0, (int)p->N[1], (int)p->N[0], sign))) return rc;
+  RowsArgs a = rows_args(p);
+  a.img_out = img;
+  a.smaps = smaps;
+  a.C = C;
+  a.t0 = t0;
+  a.T = T;
+  a.scale = scale;
+  const int rows = a.M0 * a.M1;
+  const int L = p->fft[2].L;
+  const int nimg = smaps ? ((t0 + T - 1) / C - t0 / C + 1) : T;
+  const dim3 grd((unsigned)((rows + L - 1) / L), (unsigned)nimg);
+  const size_t lds = fft_lds(p->fft[2]);
+  if (sign < 0)
+    hipLaunchKernelGGL(k_rows_adj<-1>, grd, dim3(kThreads), lds, p->stream, a);
+  else
+    hipLaunchKernelGGL(k_rows_adj<1>, grd, dim3(kThreads), lds, p->stream, a);
+  HN_LAUNCH_CHECK();
+  return 0;
+}
+
+}  // namespace hn

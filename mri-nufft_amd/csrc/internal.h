@@ -1,0 +1,150 @@
+// Internal declarations of libhipnufft (gfx950).  Not part of the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "hipnufft.h"
+
+namespace hn {
+
+constexpr int kThreads = 256;     // threads per workgroup of every kernel here
+constexpr int kMaxElemsFFT = 4096;  // complex elements per FFT workgroup (16 per thread)
+constexpr int kMaxRadices = 16;
+
+// Record an error message for hipnufft_last_error().
+int set_error(int code, const std::string& msg);
+
+#define HN_HIP(call)                                                        \
+  do {                                                                      \
+    hipError_t e_ = (call);                                                 \
+    if (e_ != hipSuccess)                                                   \
+      return ::hn::set_error(HIPNUFFT_ERR_HIP,                              \
+                             std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define HN_LAUNCH_CHECK()                                                   \
+  do {                                                                      \
+    hipError_t e_ = hipGetLastError();                                      \
+    if (e_ != hipSuccess)                                                   \
+      return ::hn::set_error(HIPNUFFT_ERR_HIP,                              \
+                             std::string("kernel launch: ") + hipGetErrorString(e_)); \
+  } while (0)
+
+// One FFT axis: length, radix plan, twiddle table, lines per workgroup.
+struct FFTAxis {
+  int n = 1;
+  int L = 1;          // lines per workgroup (power of two)
+  int nrad = 0;
+  int rad[kMaxRadices] = {0};
+  float2* tw = nullptr;  // tw[t] = exp(-2 pi i t / n), t < n
+};
+
+// Geometry of the points / tiles for spreading and interpolation.  Arrays
+// are indexed by the D actual dimensions (0 = slowest).
+struct PtsGeom {
+  int D = 0;
+  int nf[3] = {1, 1, 1};
+  int tile[3] = {1, 1, 1};
+  int nt[3] = {1, 1, 1};
+  int w = 1;
+  float beta = 0.f;
+  int64_t M = 0;
+  const int32_t* cell[3] = {nullptr, nullptr, nullptr};  // sorted order
+  const float* off[3] = {nullptr, nullptr, nullptr};     // sorted order
+  const int32_t* idx = nullptr;                          // sorted -> original
+};
+
+}  // namespace hn
+
+struct hipnufft_plan_s {
+  int type = 2, ndim = 0, iflag = -1, ntrans = 1;
+  int sign2 = -1;  // sign of the type-2 (forward) exponent
+  double eps = 1e-6, upsampfac = 2.0;
+  int sio = 0;
+  int64_t N[3] = {1, 1, 1};   // modes, 3-padded C order (leading 1s)
+  int64_t nf[3] = {1, 1, 1};  // fine grid, 3-padded
+  int w = 1;
+  double beta = 0.0;
+  int tile[3] = {1, 1, 1};    // 3-padded
+  int nt[3] = {1, 1, 1};      // tiles per axis, 3-padded
+  int64_t nbins = 1;
+  int spread_group = 1, interp_group = 1, max_sub = 1024;
+  hipStream_t stream = nullptr;
+
+  float* dec[3] = {nullptr, nullptr, nullptr};  // deconvolution per padded axis
+  hn::FFTAxis fft[3];
+
+  // sorted points
+  int64_t M = 0;
+  int has_pts = 0;
+  int32_t* pt_cell[3] = {nullptr, nullptr, nullptr};
+  float* pt_off[3] = {nullptr, nullptr, nullptr};
+  int32_t* pt_idx = nullptr;
+  int32_t* bin_start = nullptr;   // nbins + 1
+  int32_t* sub_tile = nullptr;    // interp work items
+  int32_t* sub_begin = nullptr;
+  int64_t nsub = 0;
+  int32_t* list_pts = nullptr;    // spread lists (sorted-point positions)
+  int32_t* list_start = nullptr;  // nbins + 1
+  int64_t nlist = 0;
+  int64_t M_cap = 0, list_cap = 0, sub_cap = 0;
+
+  // workspace
+  float2* grid = nullptr;   // ntrans * prod(nf)
+  float2* kwork = nullptr;  // ntrans * M (data-consistency residual)
+  int64_t kwork_cap = 0;
+  size_t bytes = 0;
+};
+
+namespace hn {
+
+int64_t grid_size(const hipnufft_plan_s* p);
+PtsGeom pts_geom(const hipnufft_plan_s* p);
+
+// ---- sort.hip -------------------------------------------------------------
+// Bins the points and builds the sorted arrays, interp work items and
+// spread lists of the plan.  Synchronises the plan stream.
+int build_points(hipnufft_plan_s* p, int64_t M, const float* const* xd);
+
+// ---- spread_interp.hip ---------------------------------------------------
+// Interpolation over transforms [0, T): out[t][idx] (original order).
+// mode 0: out = scale * interp
+// mode 1: out = (scale * interp - obs[t][idx]) * (density ? density[idx] : 1)
+int launch_interp(const hipnufft_plan_s* p, const float2* grid,
+                  int64_t grid_stride, int T, float2* out, const float2* obs,
+                  const float* density, float scale, int mode);
+// Spreading of in[t][idx] * (density ? density[idx] : 1) for t < T into the
+// grid (every cell of every transform written once; no pre-zeroing).
+int launch_spread(const hipnufft_plan_s* p, const float2* in,
+                  const float* density, int T, float2* grid,
+                  int64_t grid_stride);
+// Trajectory gradient by the kernel derivative (type-2 direction).
+int launch_grad_traj(const hipnufft_plan_s* p, const float2* grid,
+                     int64_t grid_stride, int T, const float2* dy,
+                     float scale, float* gtraj);
+
+// ---- fft.hip -------------------------------------------------------------
+// Forward chain image(s) -> fine grid (deconvolve, pad, all FFT passes).
+// Transform t of the chunk uses image (smaps ? (t0+t)/C : t0+t) and coil
+// (t0+t)%C.
+int fwd_to_grid(const hipnufft_plan_s* p, const float2* img,
+                const float2* smaps, int C, int t0, int T, float scale,
+                int sign, int conj_smaps = 0);
+// Adjoint chain fine grid -> image(s) (all FFT passes, crop, deconvolve and,
+// with smaps, conjugate coil combine).  With smaps the chunk [t0, t0+T)
+// covers batches t0/C ... ; accumulate adds into img instead of writing.
+int grid_to_img(const hipnufft_plan_s* p, float2* img, const float2* smaps,
+                int C, int t0, int T, float scale, int sign);
+
+int dot_c64(const float2* a, const float2* b, int64_t n, int conj,
+            double* out, double* ws, hipStream_t s);
+
+// Exclusive scan of n int32 (device), result in out (may alias in), total in
+// *total (host, synchronises the stream) if total != nullptr.
+int scan_exclusive(const int32_t* in, int32_t* out, int64_t n,
+                   hipStream_t s, int64_t* total);
+
+}  // namespace hn

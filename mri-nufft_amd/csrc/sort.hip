@@ -1,0 +1,569 @@
+// Bit-exact binning and stable sort of the non-uniform points (setpts).
+//
+// Replaces the bin-sort that cufinufft performs inside plan._setpts
+// (operators/interfaces/cufinufft.py:94-113, gpu_sort=1, cufinufft.py:38-46).
+// Every rule here is integer or a single IEEE double operation, evaluated with
+// floating-point contraction disabled, so the permutation equals the CPU
+// oracle's numpy.argsort(kind="stable") bit for bit.
+#pragma clang fp contract(off)
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <vector>
+
+#include "internal.h"
+
+namespace hn {
+namespace {
+
+constexpr int kScanIPT = 8;                 // items per thread in scans
+constexpr int kScanChunk = kThreads * kScanIPT;
+constexpr int kSortIPT = 16;                // items per thread in radix passes
+constexpr int kSortChunk = kThreads * kSortIPT;
+constexpr int kWaves = kThreads / 64;
+
+// ---------------------------------------------------------------------------
+// exclusive scan
+// ---------------------------------------------------------------------------
+__device__ int block_exclusive_scan(int v, int* wsum, int* total) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) wsum[wave] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int run = 0;
+    for (int w = 0; w < kWaves; ++w) {
+      int t = wsum[w];
+      wsum[w] = run;
+      run += t;
+    }
+    *total = run;
+  }
+  __syncthreads();
+  return x - v + wsum[wave];
+}
+
+__global__ __launch_bounds__(kThreads) void k_scan_blocks(const int32_t* in, int32_t* out,
+                                                          int32_t* bsum, int64_t n) {
+  __shared__ int wsum[kWaves];
+  __shared__ int total;
+  const int64_t base = (int64_t)blockIdx.x * kScanChunk + (int64_t)threadIdx.x * kScanIPT;
+  int v[kScanIPT];
+  int s = 0;
+#pragma unroll
+  for (int i = 0; i < kScanIPT; ++i) {
+    v[i] = (base + i < n) ? in[base + i] : 0;
+    s += v[i];
+  }
+  int ex = block_exclusive_scan(s, wsum, &total);
+#pragma unroll
+  for (int i = 0; i < kScanIPT; ++i) {
+    if (base + i < n) out[base + i] = ex;
+    ex += v[i];
+  }
+  if (threadIdx.x == 0 && bsum) bsum[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(kThreads) void k_scan_add(int32_t* out, const int32_t* boff, int64_t n) {
+  const int64_t base = (int64_t)blockIdx.x * kScanChunk;
+  const int add = boff[blockIdx.x];
+  for (int i = threadIdx.x; i < kScanChunk; i += kThreads)
+    if (base + i < n) out[base + i] += add;
+}
+
+// ---------------------------------------------------------------------------
+// point folding / binning
+// ---------------------------------------------------------------------------
+struct FoldArgs {
+  const float* x[3];
+  double scale[3];
+  double nf[3];
+  double inv_tile[3];
+  int nt[3];
+  int nfi[3];
+  double halfw;
+  int64_t M;
+  uint32_t* key;
+  int32_t* val;
+  int32_t* cell[3];
+  float* off[3];
+};
+
+__device__ inline double fold_coord(float x, double scale, double nf) {
+  double g = (double)x * scale;
+  if (g < 0.0) g = g + nf;
+  if (g < 0.0) g = g + nf;
+  if (g >= nf) g = g - nf;
+  if (g >= nf) g = g - nf;
+  if (!(g >= 0.0 && g < nf)) g = 0.0;  // NaN / far out of range: defined, in bounds
+  return g;
+}
+
+template <int D>
+__global__ __launch_bounds__(kThreads) void k_fold_bin(FoldArgs a) {
+  const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (i >= a.M) return;
+  uint32_t key = 0;
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    const double g = fold_coord(a.x[d][i], a.scale[d], a.nf[d]);
+    int b = (int)floor(g * a.inv_tile[d]);
+    if (b >= a.nt[d]) b = a.nt[d] - 1;
+    key = key * (uint32_t)a.nt[d] + (uint32_t)b;
+    const double s = ceil(g - a.halfw);
+    int c0 = (int)s;
+    a.off[d][i] = (float)(g - s);
+    if (c0 < 0) c0 += a.nfi[d];
+    if (c0 >= a.nfi[d]) c0 -= a.nfi[d];
+    a.cell[d][i] = c0;
+  }
+  a.key[i] = key;
+  a.val[i] = (int32_t)i;
+}
+
+// ---------------------------------------------------------------------------
+// stable LSD radix sort, 8-bit digits
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kThreads) void k_digit_hist(const uint32_t* keys, int64_t n, int shift,
+                                                         int32_t* hist, int nblk) {
+  __shared__ int h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const int64_t base = (int64_t)blockIdx.x * kSortChunk;
+  for (int r = 0; r < kSortIPT; ++r) {
+    const int64_t i = base + r * kThreads + threadIdx.x;
+    if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1);
+  }
+  __syncthreads();
+  hist[(int64_t)threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
+}
+
+__global__ __launch_bounds__(kThreads) void k_digit_scatter(const uint32_t* kin, const int32_t* vin,
+                                                            uint32_t* kout, int32_t* vout,
+                                                            int64_t n, int shift,
+                                                            const int32_t* offs, int nblk) {
+  __shared__ int base[256];
+  __shared__ int wcnt[kWaves][256];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const unsigned long long lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+  base[threadIdx.x] = offs[(int64_t)threadIdx.x * nblk + blockIdx.x];
+  const int64_t b0 = (int64_t)blockIdx.x * kSortChunk;
+  for (int r = 0; r < kSortIPT; ++r) {
+    const int64_t i = b0 + r * kThreads + threadIdx.x;
+    const bool valid = i < n;
+    const uint32_t key = valid ? kin[i] : 0u;
+    const int dig = (int)((key >> shift) & 255u);
+    for (int k = threadIdx.x; k < kWaves * 256; k += kThreads) (&wcnt[0][0])[k] = 0;
+    __syncthreads();
+    unsigned long long peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const bool bit = (dig >> b) & 1;
+      const unsigned long long m = __ballot(bit);
+      peers &= bit ? m : ~m;
+    }
+    const int lrank = __popcll(peers & lt);
+    if (valid && lrank == 0) wcnt[wave][dig] = __popcll(peers);
+    __syncthreads();
+    {
+      int run = base[threadIdx.x];
+      for (int w = 0; w < kWaves; ++w) {
+        const int c = wcnt[w][threadIdx.x];
+        wcnt[w][threadIdx.x] = run;
+        run += c;
+      }
+      base[threadIdx.x] = run;
+    }
+    __syncthreads();
+    if (valid) {
+      const int pos = wcnt[wave][dig] + lrank;
+      kout[pos] = key;
+      vout[pos] = vin[i];
+    }
+    __syncthreads();
+  }
+}
+
+// Sort (keys, vals) stably by key < 2^bits.  Result ends in (k0, v0).
+int radix_sort(uint32_t* k0, int32_t* v0, uint32_t* k1, int32_t* v1, int64_t n,
+               int bits, hipStream_t s) {
+  if (n <= 1 || bits <= 0) return 0;
+  const int nblk = (int)((n + kSortChunk - 1) / kSortChunk);
+  int32_t* hist = nullptr;
+  HN_HIP(hipMallocAsync((void**)&hist, sizeof(int32_t) * 256 * (size_t)nblk, s));
+  for (int shift = 0; shift < bits; shift += 8) {
+    hipLaunchKernelGGL(k_digit_hist, dim3(nblk), dim3(kThreads), 0, s, k0, n, shift, hist, nblk);
+    HN_LAUNCH_CHECK();
+    int rc = scan_exclusive(hist, hist, 256LL * nblk, s, nullptr);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_digit_scatter, dim3(nblk), dim3(kThreads), 0, s, k0, v0, k1, v1, n,
+                       shift, hist, nblk);
+    HN_LAUNCH_CHECK();
+    std::swap(k0, k1);
+    std::swap(v0, v1);
+  }
+  HN_HIP(hipFreeAsync(hist, s));
+  // an odd number of passes leaves the result in the second buffer: copy back
+  const int passes = (bits + 7) / 8;
+  if (passes & 1) {
+    HN_HIP(hipMemcpyAsync(k1, k0, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, s));
+    HN_HIP(hipMemcpyAsync(v1, v0, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, s));
+  }
+  return 0;
+}
+
+// start[b] = first sorted position with key >= b, b in [0, nbins]
+__global__ __launch_bounds__(kThreads) void k_bin_start(const uint32_t* skeys, int64_t n, int64_t nbins,
+                                                        int32_t* start) {
+  const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (i > n) return;
+  const int64_t prev = (i == 0) ? -1 : (int64_t)skeys[i - 1];
+  const int64_t cur = (i == n) ? nbins : (int64_t)skeys[i];
+  for (int64_t b = prev + 1; b <= cur; ++b) start[b] = (int32_t)i;
+}
+
+struct PermArgs {
+  const int32_t* perm;
+  int64_t M;
+  const int32_t* cin[3];
+  const float* oin[3];
+  int32_t* cout[3];
+  float* oout[3];
+  int32_t* idx;
+};
+
+template <int D>
+__global__ __launch_bounds__(kThreads) void k_permute(PermArgs a) {
+  const int64_t p = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (p >= a.M) return;
+  const int32_t o = a.perm[p];
+  a.idx[p] = o;
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    a.cout[d][p] = a.cin[d][o];
+    a.oout[d][p] = a.oin[d][o];
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void k_sub_count(const int32_t* bs, int64_t nbins, int S,
+                                                        int32_t* cnt) {
+  const int64_t b = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (b >= nbins) return;
+  const int n = bs[b + 1] - bs[b];
+  cnt[b] = (n + S - 1) / S;
+}
+
+__global__ __launch_bounds__(kThreads) void k_sub_emit(const int32_t* bs, const int32_t* soff,
+                                                       int64_t nbins, int S, int32_t* st,
+                                                       int32_t* sb) {
+  const int64_t b = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (b >= nbins) return;
+  const int n = bs[b + 1] - bs[b];
+  const int k = (n + S - 1) / S;
+  for (int s = 0; s < k; ++s) {
+    st[soff[b] + s] = (int32_t)b;
+    sb[soff[b] + s] = bs[b] + s * S;
+  }
+}
+
+// distinct tiles touched along one axis, in order of first appearance
+__device__ inline int axis_tiles(int c0, int w, int nf, int tile, int* t) {
+  int n = 0;
+  for (int j = 0; j < w; ++j) {
+    int c = c0 + j;
+    if (c >= nf) c -= nf;
+    const int tt = c / tile;
+    bool seen = false;
+    for (int q = 0; q < n; ++q) seen |= (t[q] == tt);
+    if (!seen && n < 4) t[n++] = tt;  // tile >= w/2 bounds this at 3
+  }
+  return n;
+}
+
+struct ListArgs {
+  const int32_t* cell[3];
+  int nf[3];
+  int tile[3];
+  int nt[3];
+  int w;
+  int64_t M;
+  const int32_t* eoff;
+  int32_t* cnt;
+  uint32_t* lkey;
+  int32_t* lval;
+};
+
+template <int D>
+__global__ __launch_bounds__(kThreads) void k_list_count(ListArgs a) {
+  const int64_t p = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (p >= a.M) return;
+  int total = 1;
+  for (int d = 0; d < D; ++d) {
+    int t[4];
+    total *= axis_tiles(a.cell[d][p], a.w, a.nf[d], a.tile[d], t);
+  }
+  a.cnt[p] = total;
+}
+
+template <int D>
+__global__ __launch_bounds__(kThreads) void k_list_emit(ListArgs a) {
+  const int64_t p = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (p >= a.M) return;
+  int t[3][4];
+  int n[3] = {1, 1, 1};
+  for (int d = 0; d < D; ++d) n[d] = axis_tiles(a.cell[d][p], a.w, a.nf[d], a.tile[d], t[d]);
+  int e = a.eoff[p];
+  for (int i0 = 0; i0 < n[0]; ++i0)
+    for (int i1 = 0; i1 < (D > 1 ? n[1] : 1); ++i1)
+      for (int i2 = 0; i2 < (D > 2 ? n[2] : 1); ++i2) {
+        uint32_t key = (uint32_t)t[0][i0];
+        if (D > 1) key = key * (uint32_t)a.nt[1] + (uint32_t)t[1][i1];
+        if (D > 2) key = key * (uint32_t)a.nt[2] + (uint32_t)t[2][i2];
+        a.lkey[e] = key;
+        a.lval[e] = (int32_t)p;
+        ++e;
+      }
+}
+
+int bits_for(int64_t nbins) {
+  int b = 0;
+  while (((int64_t)1 << b) < nbins) ++b;
+  return b;
+}
+
+template <typename T>
+int grow(T** ptr, int64_t* cap, int64_t need) {
+  if (need <= *cap && *ptr) return 0;
+  if (*ptr) HN_HIP(hipFree(*ptr));
+  *ptr = nullptr;
+  const int64_t n = need > 0 ? need : 1;
+  HN_HIP(hipMalloc((void**)ptr, sizeof(T) * n));
+  *cap = n;
+  return 0;
+}
+
+}  // namespace
+
+int scan_exclusive(const int32_t* in, int32_t* out, int64_t n, hipStream_t s, int64_t* total) {
+  if (n <= 0) {
+    if (total) *total = 0;
+    return 0;
+  }
+  const int64_t nblk = (n + kScanChunk - 1) / kScanChunk;
+  // the last input element is read before the (possibly in-place) scan
+  int32_t last_in = 0;
+  if (total)
+    HN_HIP(hipMemcpyAsync(&last_in, in + n - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  int32_t* bsum = nullptr;
+  HN_HIP(hipMallocAsync((void**)&bsum, sizeof(int32_t) * (nblk + 1), s));
+  hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nblk), dim3(kThreads), 0, s, in, out, bsum, n);
+  HN_LAUNCH_CHECK();
+  if (nblk > 1) {
+    int rc = scan_exclusive(bsum, bsum, nblk, s, nullptr);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nblk), dim3(kThreads), 0, s, out, bsum, n);
+    HN_LAUNCH_CHECK();
+  } else {
+    int32_t z = 0;
+    HN_HIP(hipMemcpyAsync(bsum, &z, sizeof(int32_t), hipMemcpyHostToDevice, s));
+  }
+  if (total) {
+    int32_t last_out = 0;
+    HN_HIP(hipMemcpyAsync(&last_out, out + n - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HN_HIP(hipStreamSynchronize(s));
+    *total = (int64_t)last_in + last_out;
+  }
+  HN_HIP(hipFreeAsync(bsum, s));
+  return 0;
+}
+
+int build_points(hipnufft_plan_s* p, int64_t M, const float* const* xd) {
+  const int D = p->ndim;
+  const int o = 3 - D;  // first real axis in the padded arrays
+  hipStream_t s = p->stream;
+  int rc;
+  if (M > INT32_MAX / 2) return set_error(HIPNUFFT_ERR_ARG, "too many points (max 2^30)");
+  for (int d = 0; d < D; ++d)
+    if (!xd[d]) return set_error(HIPNUFFT_ERR_ARG, "missing coordinate array");
+
+  // persistent sorted arrays (one capacity for all of them)
+  if (M > p->M_cap || !p->pt_idx) {
+    for (int d = 0; d < 3; ++d) {
+      if (p->pt_cell[d]) HN_HIP(hipFree(p->pt_cell[d]));
+      if (p->pt_off[d]) HN_HIP(hipFree(p->pt_off[d]));
+      p->pt_cell[d] = nullptr;
+      p->pt_off[d] = nullptr;
+    }
+    if (p->pt_idx) HN_HIP(hipFree(p->pt_idx));
+    p->pt_idx = nullptr;
+    const int64_t cap = M > 0 ? M : 1;
+    for (int d = 0; d < D; ++d) {
+      HN_HIP(hipMalloc((void**)&p->pt_cell[d], sizeof(int32_t) * cap));
+      HN_HIP(hipMalloc((void**)&p->pt_off[d], sizeof(float) * cap));
+    }
+    HN_HIP(hipMalloc((void**)&p->pt_idx, sizeof(int32_t) * cap));
+    p->M_cap = cap;
+  }
+  p->M = M;
+  p->has_pts = 1;
+
+  const int64_t nbins = p->nbins;
+  if (!p->bin_start) HN_HIP(hipMalloc((void**)&p->bin_start, sizeof(int32_t) * (nbins + 1)));
+  if (!p->list_start) HN_HIP(hipMalloc((void**)&p->list_start, sizeof(int32_t) * (nbins + 1)));
+
+  if (M == 0) {
+    HN_HIP(hipMemsetAsync(p->bin_start, 0, sizeof(int32_t) * (nbins + 1), s));
+    HN_HIP(hipMemsetAsync(p->list_start, 0, sizeof(int32_t) * (nbins + 1), s));
+    p->nsub = 0;
+    p->nlist = 0;
+    HN_HIP(hipStreamSynchronize(s));
+    return 0;
+  }
+
+  // temporaries (original order)
+  uint32_t *ka = nullptr, *kb = nullptr;
+  int32_t *va = nullptr, *vb = nullptr;
+  int32_t* ctmp[3] = {nullptr, nullptr, nullptr};
+  float* otmp[3] = {nullptr, nullptr, nullptr};
+  HN_HIP(hipMallocAsync((void**)&ka, sizeof(uint32_t) * M, s));
+  HN_HIP(hipMallocAsync((void**)&kb, sizeof(uint32_t) * M, s));
+  HN_HIP(hipMallocAsync((void**)&va, sizeof(int32_t) * M, s));
+  HN_HIP(hipMallocAsync((void**)&vb, sizeof(int32_t) * M, s));
+  for (int d = 0; d < D; ++d) {
+    HN_HIP(hipMallocAsync((void**)&ctmp[d], sizeof(int32_t) * M, s));
+    HN_HIP(hipMallocAsync((void**)&otmp[d], sizeof(float) * M, s));
+  }
+
+  FoldArgs fa{};
+  for (int d = 0; d < D; ++d) {
+    const int64_t nf = p->nf[o + d];
+    fa.x[d] = xd[d];
+    fa.scale[d] = (double)nf / (2.0 * M_PI);
+    fa.nf[d] = (double)nf;
+    fa.nfi[d] = (int)nf;
+    fa.inv_tile[d] = 1.0 / (double)p->tile[o + d];
+    fa.nt[d] = p->nt[o + d];
+    fa.cell[d] = ctmp[d];
+    fa.off[d] = otmp[d];
+  }
+  fa.halfw = 0.5 * p->w;
+  fa.M = M;
+  fa.key = ka;
+  fa.val = va;
+  const unsigned nb = (unsigned)((M + kThreads - 1) / kThreads);
+  if (D == 1) hipLaunchKernelGGL(k_fold_bin<1>, dim3(nb), dim3(kThreads), 0, s, fa);
+  if (D == 2) hipLaunchKernelGGL(k_fold_bin<2>, dim3(nb), dim3(kThreads), 0, s, fa);
+  if (D == 3) hipLaunchKernelGGL(k_fold_bin<3>, dim3(nb), dim3(kThreads), 0, s, fa);
+  HN_LAUNCH_CHECK();
+
+  const int bits = bits_for(nbins);
+  if ((rc = radix_sort(ka, va, kb, vb, M, bits, s))) return rc;
+
+  {
+    const unsigned g = (unsigned)((M + 1 + kThreads - 1) / kThreads);
+    hipLaunchKernelGGL(k_bin_start, dim3(g), dim3(kThreads), 0, s, ka, M, nbins, p->bin_start);
+    HN_LAUNCH_CHECK();
+  }
+  {
+    PermArgs pa{};
+    pa.perm = va;
+    pa.M = M;
+    for (int d = 0; d < D; ++d) {
+      pa.cin[d] = ctmp[d];
+      pa.oin[d] = otmp[d];
+      pa.cout[d] = p->pt_cell[d];
+      pa.oout[d] = p->pt_off[d];
+    }
+    pa.idx = p->pt_idx;
+    if (D == 1) hipLaunchKernelGGL(k_permute<1>, dim3(nb), dim3(kThreads), 0, s, pa);
+    if (D == 2) hipLaunchKernelGGL(k_permute<2>, dim3(nb), dim3(kThreads), 0, s, pa);
+    if (D == 3) hipLaunchKernelGGL(k_permute<3>, dim3(nb), dim3(kThreads), 0, s, pa);
+    HN_LAUNCH_CHECK();
+  }
+
+  // interp work items: split dense bins into chunks of max_sub points
+  {
+    int32_t* cnt = nullptr;
+    HN_HIP(hipMallocAsync((void**)&cnt, sizeof(int32_t) * nbins, s));
+    const unsigned g = (unsigned)((nbins + kThreads - 1) / kThreads);
+    hipLaunchKernelGGL(k_sub_count, dim3(g), dim3(kThreads), 0, s, p->bin_start, nbins, p->max_sub,
+                       cnt);
+    HN_LAUNCH_CHECK();
+    int64_t nsub = 0;
+    if ((rc = scan_exclusive(cnt, cnt, nbins, s, &nsub))) return rc;
+    if (nsub > p->sub_cap || !p->sub_tile) {
+      int64_t c1 = 0, c2 = 0;
+      if ((rc = grow(&p->sub_tile, &c1, nsub))) return rc;
+      if ((rc = grow(&p->sub_begin, &c2, nsub))) return rc;
+      p->sub_cap = c1;
+    }
+    p->nsub = nsub;
+    hipLaunchKernelGGL(k_sub_emit, dim3(g), dim3(kThreads), 0, s, p->bin_start, cnt, nbins,
+                       p->max_sub, p->sub_tile, p->sub_begin);
+    HN_LAUNCH_CHECK();
+    HN_HIP(hipFreeAsync(cnt, s));
+  }
+
+  // spread lists: one entry per (point, touched tile), stably sorted by tile
+  {
+    ListArgs la{};
+    for (int d = 0; d < D; ++d) {
+      la.cell[d] = p->pt_cell[d];
+      la.nf[d] = (int)p->nf[o + d];
+      la.tile[d] = p->tile[o + d];
+      la.nt[d] = p->nt[o + d];
+    }
+    la.w = p->w;
+    la.M = M;
+    la.cnt = vb;  // reuse temporaries: vb = counts, then offsets
+    if (D == 1) hipLaunchKernelGGL(k_list_count<1>, dim3(nb), dim3(kThreads), 0, s, la);
+    if (D == 2) hipLaunchKernelGGL(k_list_count<2>, dim3(nb), dim3(kThreads), 0, s, la);
+    if (D == 3) hipLaunchKernelGGL(k_list_count<3>, dim3(nb), dim3(kThreads), 0, s, la);
+    HN_LAUNCH_CHECK();
+    int64_t L = 0;
+    if ((rc = scan_exclusive(vb, vb, M, s, &L))) return rc;
+    if (L > INT32_MAX / 2) return set_error(HIPNUFFT_ERR_ARG, "spread lists too large");
+    uint32_t *lk0 = nullptr, *lk1 = nullptr;
+    int32_t* lv1 = nullptr;
+    HN_HIP(hipMallocAsync((void**)&lk0, sizeof(uint32_t) * L, s));
+    HN_HIP(hipMallocAsync((void**)&lk1, sizeof(uint32_t) * L, s));
+    HN_HIP(hipMallocAsync((void**)&lv1, sizeof(int32_t) * L, s));
+    int64_t c = p->list_cap;
+    if ((rc = grow(&p->list_pts, &c, L))) return rc;
+    p->list_cap = c;
+    p->nlist = L;
+    la.eoff = vb;
+    la.lkey = lk0;
+    la.lval = p->list_pts;
+    if (D == 1) hipLaunchKernelGGL(k_list_emit<1>, dim3(nb), dim3(kThreads), 0, s, la);
+    if (D == 2) hipLaunchKernelGGL(k_list_emit<2>, dim3(nb), dim3(kThreads), 0, s, la);
+    if (D == 3) hipLaunchKernelGGL(k_list_emit<3>, dim3(nb), dim3(kThreads), 0, s, la);
+    HN_LAUNCH_CHECK();
+    if ((rc = radix_sort(lk0, p->list_pts, lk1, lv1, L, bits, s))) return rc;
+    const unsigned g = (unsigned)((L + 1 + kThreads - 1) / kThreads);
+    hipLaunchKernelGGL(k_bin_start, dim3(g), dim3(kThreads), 0, s, lk0, L, nbins, p->list_start);
+    HN_LAUNCH_CHECK();
+    HN_HIP(hipFreeAsync(lk0, s));
+    HN_HIP(hipFreeAsync(lk1, s));
+    HN_HIP(hipFreeAsync(lv1, s));
+  }
+
+  HN_HIP(hipFreeAsync(ka, s));
+  HN_HIP(hipFreeAsync(kb, s));
+  HN_HIP(hipFreeAsync(va, s));
+  HN_HIP(hipFreeAsync(vb, s));
+  for (int d = 0; d < D; ++d) {
+    HN_HIP(hipFreeAsync(ctmp[d], s));
+    HN_HIP(hipFreeAsync(otmp[d], s));
+  }
+  HN_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+}  // namespace hn

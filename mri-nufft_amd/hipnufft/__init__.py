@@ -1,0 +1,51 @@
+"""hipnufft: an MI355X-native NUFFT backend with mri-nufft's operator interface.
+
+``get_operator("hipnufft", samples, shape, density=..., n_coils=..., smaps=...)``
+returns an operator with mri-nufft's ``op`` / ``adj_op`` / ``data_consistency``
+/ ``pinv_solver`` / ``get_lipschitz_cst`` surface; the arithmetic runs in
+``libhipnufft.so`` (hand-written gfx950 HIP kernels) reached through a ctypes
+C ABI on PyTorch-ROCm device tensors.  There is no CPU path: on a machine
+without the library or a GPU the backend is registered as unavailable and
+every transform raises.
+"""
+
+from .base import FourierOperatorBase, check_backend, get_operator, list_backends, power_method
+from .density import get_density, pipe
+from .operator import MRIHipNUFFT
+from .optim import cg, get_optimizer, lsmr, lsqr
+
+__all__ = [
+    "FourierOperatorBase",
+    "MRIHipNUFFT",
+    "cg",
+    "check_backend",
+    "get_density",
+    "get_operator",
+    "get_optimizer",
+    "list_backends",
+    "lsmr",
+    "lsqr",
+    "pipe",
+    "power_method",
+    "register_into_mrinufft",
+]
+
+
+def register_into_mrinufft():
+    """Register the backend in an installed ``mrinufft`` (no-op if absent).
+
+    Defines ``mrinufft.operators.interfaces.hipnufft.MRIHipNUFFT`` as a
+    subclass of mrinufft's ``FourierOperatorBase`` so that
+    ``mrinufft.get_operator("hipnufft", ...)`` returns this backend.
+    """
+    try:
+        import mrinufft.operators.base as mb
+    except ImportError:
+        return False
+    if "hipnufft" in mb.FourierOperatorBase.interfaces:
+        return True
+    # the class body registers itself through FourierOperatorBase.__init_subclass__
+    type("MRIHipNUFFT", (MRIHipNUFFT, mb.FourierOperatorBase),
+         {"backend": "hipnufft", "available": MRIHipNUFFT.available,
+          "__module__": "mrinufft.operators.interfaces.hipnufft"})
+    return True

@@ -1,0 +1,262 @@
+"""ctypes binding of ``libhipnufft.so`` (the C ABI declared in include/hipnufft.h).
+
+The library links against ``libamdhip64.so.7``; it is loaded only after
+``torch`` so that it binds to the HIP runtime torch already loaded (one HIP
+runtime per process, shared device pointers and streams).  There is no CPU
+fallback: if the library is missing or the device is absent every call raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import threading
+from pathlib import Path
+
+import torch  # noqa: F401  (must be imported before the library is loaded)
+
+LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libhipnufft.so"
+
+ERR_ARG = -1
+ERR_HIP = -2
+ERR_NOPTS = -3
+ERR_ALLOC = -4
+
+_lock = threading.Lock()
+_lib = None
+
+
+class HipNufftError(RuntimeError):
+    """Failure reported by libhipnufft (HIP error, allocation failure...)."""
+
+
+class Opts(ctypes.Structure):
+    """Mirror of ``hipnufft_opts``."""
+
+    _fields_ = [
+        ("upsampfac", ctypes.c_double),
+        ("spreadinterponly", ctypes.c_int),
+        ("tile", ctypes.c_int * 3),
+        ("spread_group", ctypes.c_int),
+        ("interp_group", ctypes.c_int),
+        ("max_subproblem", ctypes.c_int),
+        ("deterministic", ctypes.c_int),
+    ]
+
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_I64 = ctypes.c_int64
+_F = ctypes.c_float
+
+_SIGNATURES = {
+    "hipnufft_default_opts": [ctypes.POINTER(Opts)],
+    "hipnufft_plan_create": [_I, _I, ctypes.POINTER(_I64), _I, _I, ctypes.c_double,
+                             ctypes.POINTER(Opts), _P, ctypes.POINTER(_P)],
+    "hipnufft_setpts": [_P, _I64, _P, _P, _P],
+    "hipnufft_execute": [_P, _P, _P],
+    "hipnufft_execute_adjoint": [_P, _P, _P],
+    "hipnufft_sense_op": [_P, _P, _P, _P, _I, _I, _F],
+    "hipnufft_sense_adj": [_P, _P, _P, _P, _P, _I, _I, _F],
+    "hipnufft_sense_dc": [_P, _P, _P, _P, _P, _P, _I, _I, _F],
+    "hipnufft_sense_grad_traj": [_P, _P, _P, _P, _I, _I, _F, _I, _I, _P],
+    "hipnufft_dot_c64": [_P, _P, _I64, _I, _P, _P, _P],
+    "hipnufft_get_sort": [_P, _P, _P, _P, _P, ctypes.POINTER(_I64)],
+    "hipnufft_plan_info": [_P, ctypes.POINTER(_I64), ctypes.POINTER(_I),
+                           ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_I),
+                           ctypes.POINTER(_I64)],
+    "hipnufft_workspace_bytes": [_P, ctypes.POINTER(ctypes.c_size_t)],
+    "hipnufft_set_stream": [_P, _P],
+    "hipnufft_plan_destroy": [_P],
+}
+
+EXPORTED = sorted(list(_SIGNATURES) + ["hipnufft_last_error"])
+
+
+def lib():
+    """Load the library once (raises if it has not been built)."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not LIB_PATH.exists():
+                raise HipNufftError(
+                    f"{LIB_PATH} is missing: build it with `python -m hipnufft._build` "
+                    "(or __graft_entry__.build())"
+                )
+            h = ctypes.CDLL(str(LIB_PATH))
+            for name, args in _SIGNATURES.items():
+                fn = getattr(h, name)
+                fn.argtypes = args
+                fn.restype = _I
+            h.hipnufft_last_error.argtypes = []
+            h.hipnufft_last_error.restype = ctypes.c_char_p
+            _lib = h
+    return _lib
+
+
+def check(rc: int, what: str = ""):
+    """Raise ValueError for argument errors and HipNufftError otherwise."""
+    if rc == 0:
+        return
+    msg = lib().hipnufft_last_error().decode(errors="replace")
+    text = f"{what}: {msg}" if what else msg
+    if rc == ERR_ARG or rc == ERR_NOPTS:
+        raise ValueError(text)
+    raise HipNufftError(text)
+
+
+def ptr(t) -> int | None:
+    """Device pointer of a tensor (None for None)."""
+    return None if t is None else t.data_ptr()
+
+
+def current_stream() -> int:
+    return torch.cuda.current_stream().cuda_stream
+
+
+class RawPlan:
+    """Owns one ``hipnufft_plan`` (both transform directions).
+
+    Mirrors ``RawCufinufftPlan`` (``operators/interfaces/cufinufft.py:51-137``)
+    with a single plan object for type 1 and type 2, like finufft's
+    ``Plan.execute`` / ``execute_adjoint`` (``interfaces/finufft.py:64-76``).
+    """
+
+    def __init__(self, shape, n_trans=1, eps=1e-6, isign=-1, upsampfac=2.0,
+                 spreadinterponly=False, tile=None, spread_group=0,
+                 interp_group=0, max_subproblem=0, stream=None):
+        L = lib()
+        self.shape = tuple(int(s) for s in shape)
+        self.ndim = len(self.shape)
+        self.n_trans = int(n_trans)
+        self.eps = float(eps)
+        self.isign = int(isign)
+        o = Opts()
+        check(L.hipnufft_default_opts(ctypes.byref(o)))
+        o.upsampfac = float(upsampfac)
+        o.spreadinterponly = 1 if spreadinterponly else 0
+        if tile is not None:
+            for d, t in enumerate(tile):
+                o.tile[d] = int(t)
+        o.spread_group = int(spread_group)
+        o.interp_group = int(interp_group)
+        o.max_subproblem = int(max_subproblem)
+        self.opts = o
+        nm = (_I64 * self.ndim)(*self.shape)
+        h = _P()
+        self.stream = current_stream() if stream is None else stream
+        check(L.hipnufft_plan_create(2, self.ndim, nm, self.isign, self.n_trans, self.eps,
+                                     ctypes.byref(o), self.stream, ctypes.byref(h)),
+              "plan_create")
+        self._h = h
+        self.n_samples = 0
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set_stream(self, stream):
+        self.stream = stream
+        check(lib().hipnufft_set_stream(self._h, stream))
+
+    def _sync_stream(self):
+        s = current_stream()
+        if s != self.stream:
+            self.set_stream(s)
+
+    def setpts(self, coords):
+        """``coords``: (M, ndim) float32 radians on the device."""
+        self._sync_stream()
+        c = coords.to(torch.float32)
+        cols = [c[:, d].contiguous() for d in range(self.ndim)]
+        args = [ptr(x) for x in cols] + [None] * (3 - self.ndim)
+        check(lib().hipnufft_setpts(self._h, c.shape[0], *args), "setpts")
+        self.n_samples = int(c.shape[0])
+
+    def info(self):
+        nf = (_I64 * 3)()
+        w = _I()
+        beta = ctypes.c_double()
+        tile = (_I * 3)()
+        M = _I64()
+        check(lib().hipnufft_plan_info(self._h, nf, ctypes.byref(w), ctypes.byref(beta), tile,
+                                       ctypes.byref(M)))
+        return dict(nf=tuple(nf), w=w.value, beta=beta.value, tile=tuple(tile), M=M.value)
+
+    def workspace_bytes(self):
+        b = ctypes.c_size_t()
+        check(lib().hipnufft_workspace_bytes(self._h, ctypes.byref(b)))
+        return b.value
+
+    def get_sort(self):
+        """Permutation, bin offsets and spread lists (int32 device tensors)."""
+        sizes = (_I64 * 3)()
+        check(lib().hipnufft_get_sort(self._h, None, None, None, None, sizes))
+        M, nbins, nlist = (int(v) for v in sizes)
+        dev = torch.device("cuda")
+        perm = torch.empty(max(M, 1), dtype=torch.int32, device=dev)
+        bs = torch.empty(nbins + 1, dtype=torch.int32, device=dev)
+        lst = torch.empty(max(nlist, 1), dtype=torch.int32, device=dev)
+        ls = torch.empty(nbins + 1, dtype=torch.int32, device=dev)
+        self._sync_stream()
+        check(lib().hipnufft_get_sort(self._h, ptr(perm), ptr(bs), ptr(lst), ptr(ls), sizes))
+        torch.cuda.synchronize()
+        return perm[:M], bs, lst[:nlist], ls
+
+    # --- transforms on contiguous complex64 device tensors ------------------
+    def sense_op(self, img, smaps, ksp, B, C, scale):
+        self._sync_stream()
+        check(lib().hipnufft_sense_op(self._h, ptr(img), ptr(smaps), ptr(ksp), B, C, scale), "op")
+
+    def sense_adj(self, ksp, smaps, density, img, B, C, scale):
+        self._sync_stream()
+        check(lib().hipnufft_sense_adj(self._h, ptr(ksp), ptr(smaps), ptr(density), ptr(img), B, C,
+                                       scale), "adj_op")
+
+    def sense_dc(self, img, smaps, obs, density, grad, B, C, scale):
+        self._sync_stream()
+        check(lib().hipnufft_sense_dc(self._h, ptr(img), ptr(smaps), ptr(obs), ptr(density),
+                                      ptr(grad), B, C, scale), "data_consistency")
+
+    def sense_grad_traj(self, img, smaps, dy, B, C, scale, isign, conj_smaps, gtraj):
+        self._sync_stream()
+        check(lib().hipnufft_sense_grad_traj(self._h, ptr(img), ptr(smaps), ptr(dy), B, C, scale,
+                                             isign, 1 if conj_smaps else 0, ptr(gtraj)),
+              "grad_traj")
+
+    def execute(self, inp, out):
+        self._sync_stream()
+        check(lib().hipnufft_execute(self._h, ptr(inp), ptr(out)), "execute")
+
+    def execute_adjoint(self, inp, out):
+        self._sync_stream()
+        check(lib().hipnufft_execute_adjoint(self._h, ptr(inp), ptr(out)), "execute_adjoint")
+
+    def close(self):
+        h, self._h = getattr(self, "_h", None), None
+        if h and _lib is not None:
+            _lib.hipnufft_plan_destroy(h)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+DOT_WS = 2048
+
+
+def dot_c64(a, b, conj=True, out=None, ws=None):
+    """FP64-accumulated ``sum(conj(a) * b)`` (or ``sum(a * b)``) of complex64
+    device tensors.  Returns a (2,) float64 device tensor [re, im] without a
+    host synchronisation; ``ws`` is an optional (DOT_WS,) float64 workspace.
+    """
+    a = a.contiguous()
+    b = b.contiguous()
+    if a.numel() != b.numel():
+        raise ValueError("dot_c64: size mismatch")
+    if out is None:
+        out = torch.empty(2, dtype=torch.float64, device=a.device)
+    check(lib().hipnufft_dot_c64(ptr(a), ptr(b), a.numel(), 1 if conj else 0, ptr(out), ptr(ws),
+                                 current_stream()), "dot")
+    return out

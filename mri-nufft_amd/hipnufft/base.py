@@ -1,0 +1,412 @@
+"""Operator registry and base contract, mirroring mri-nufft's plugin interface.
+
+Restates ``src/mrinufft/operators/base.py``:
+
+* ``check_backend`` / ``list_backends`` / ``get_operator``  (:45-179)
+* ``FourierOperatorBase`` registry via ``__init_subclass__`` (:182-218),
+  ``check_shape`` (:225-260), ``_safe_squeeze`` (:262-273), the shape /
+  coil / batch / density / smaps properties (:692-835), ``norm_factor``
+  (:837-850), ``compute_density`` (:576-624), ``compute_smaps`` (:407-448),
+  ``get_lipschitz_cst`` (:626-665), ``pinv_solver`` (:667-690),
+  ``make_autograd`` / ``with_autograd`` (:536-574, :863-877),
+  ``make_linops`` (:450-490)
+* ``power_method`` (:1155-1221) and ``_ToggleGradPlanMixin`` (:1224-1238)
+
+The same registry names, error types and messages are used so that code
+written against ``mrinufft.get_operator`` keeps working; when the real
+``mrinufft`` package is importable the backend is also registered there
+(see ``hipnufft.register_into_mrinufft``).
+"""
+
+from __future__ import annotations
+
+import logging
+import warnings
+from contextlib import contextmanager
+from functools import partial
+
+import numpy as np
+import torch
+
+from ._array import kind_of
+
+DTYPE_R2C = {"float32": "complex64", "float64": "complex128"}
+
+
+def check_backend(backend_name: str):
+    """Availability flag of a registered backend (ValueError if unknown)."""
+    backend_name = backend_name.lower()
+    try:
+        return FourierOperatorBase.interfaces[backend_name][0]
+    except KeyError as e:
+        raise ValueError(f"unknown backend: '{backend_name}'") from e
+
+
+def list_backends(available_only=False):
+    """Registered backend names."""
+    return [
+        name
+        for name, (available, _) in FourierOperatorBase.interfaces.items()
+        if available or not available_only
+    ]
+
+
+def get_operator(backend_name: str, wrt_data: bool = False, wrt_traj: bool = False,
+                 paired_batch: bool = False, *args, **kwargs):
+    """Return the operator class, an instance (when args are given) or an autograd wrapper."""
+    backend_name = backend_name.lower()
+    try:
+        available, operator = FourierOperatorBase.interfaces[backend_name]
+    except KeyError as exc:
+        raise ValueError(f"backend {backend_name} does not exist") from exc
+    if not available:
+        raise ValueError(
+            f"backend {backend_name} found, but dependencies are not met."
+            f" ``pip install mri-nufft[{backend_name}]`` may solve the issue."
+        )
+    if args or kwargs:
+        operator = operator(*args, **kwargs)
+    if wrt_data or wrt_traj:
+        if isinstance(operator, partial):
+            raise ValueError("Cannot create autograd of a partial operator.")
+        if isinstance(operator, FourierOperatorBase):
+            operator = operator.make_autograd(wrt_data=wrt_data, wrt_traj=wrt_traj,
+                                              paired_batch=paired_batch)
+        else:
+            operator = partial(operator.with_autograd, wrt_data, wrt_traj, paired_batch)
+    return operator
+
+
+class FourierOperatorBase:
+    """Base Fourier operator: registry and the shared MRI contract."""
+
+    interfaces: dict = {}
+    autograd_available = False
+    backend: str
+    available: bool
+
+    def __init__(self):
+        if not self.available:
+            raise RuntimeError(f"'{self.backend}' backend is not available.")
+        self._smaps = None
+        self._density = None
+        self._n_coils = 1
+        self._n_batchs = 1
+        self.n_trans = 1
+        self.squeeze_dims = False
+        self._density_method = None
+        self._grad_wrt_data = False
+        self._grad_wrt_traj = False
+        self._grad_wrt_field_map = False
+
+    def __init_subclass__(cls):
+        super().__init_subclass__()
+        available = getattr(cls, "available", True)
+        if callable(available):
+            available = available()
+        if backend := getattr(cls, "backend", None):
+            cls.interfaces[backend] = (available, cls)
+
+    @property
+    def log(self):
+        return logging.getLogger(type(self).__module__)
+
+    # ------------------------------------------------------------------ shapes
+    def check_shape(self, *, image=None, ksp=None):
+        """ValueError when the image / k-space trailing shape does not match."""
+        if image is not None:
+            image_shape = tuple(image.shape[-len(self.shape):])
+            if image_shape != self.shape:
+                raise ValueError(
+                    f"Image shape {image_shape} is not compatible "
+                    f"with the operator shape {self.shape}"
+                )
+        if ksp is not None:
+            kspace_shape = ksp.shape[-1]
+            if kspace_shape != self.n_samples:
+                raise ValueError(
+                    f"Kspace samples {kspace_shape} is not compatible "
+                    f"with the operator samples {self.n_samples}"
+                )
+        if image is None and ksp is None:
+            raise ValueError("Nothing to check, provides image or ksp arguments")
+
+    def _safe_squeeze(self, arr):
+        """Squeeze the coil axis, then the batch axis, when they are singleton."""
+        if self.squeeze_dims:
+            for ax in (1, 0):
+                if arr.ndim > ax and arr.shape[ax] == 1:
+                    arr = arr.squeeze(ax) if isinstance(arr, torch.Tensor) else arr.squeeze(axis=ax)
+        return arr
+
+    @property
+    def uses_sense(self):
+        return self._smaps is not None
+
+    @property
+    def uses_density(self):
+        return self.density is not None
+
+    @property
+    def ndim(self):
+        return len(self._shape)
+
+    @property
+    def shape(self):
+        return self._shape
+
+    @shape.setter
+    def shape(self, shape):
+        self._shape = tuple(int(i) for i in shape)
+
+    @property
+    def n_coils(self):
+        return self._n_coils
+
+    @n_coils.setter
+    def n_coils(self, n_coils):
+        if n_coils < 1 or not int(n_coils) == n_coils:
+            raise ValueError(f"n_coils should be a positive integer, {type(n_coils)}")
+        self._n_coils = int(n_coils)
+
+    @property
+    def n_batchs(self):
+        return self._n_batchs
+
+    @n_batchs.setter
+    def n_batchs(self, n_batchs):
+        if n_batchs < 1 or not int(n_batchs) == n_batchs:
+            raise ValueError(f"n_batchs should be a positive integer, {type(n_batchs)}")
+        self._n_batchs = int(n_batchs)
+
+    @property
+    def img_full_shape(self):
+        return (self.n_batchs, (1 if self.uses_sense else self.n_coils)) + self.shape
+
+    @property
+    def ksp_full_shape(self):
+        return (self.n_batchs, self.n_coils, self.n_samples)
+
+    @property
+    def density(self):
+        return self._density
+
+    @density.setter
+    def density(self, new_density):
+        if new_density is None:
+            self._density = None
+        elif len(new_density) != self.n_samples:
+            raise ValueError("Density and samples should have the same length")
+        else:
+            self._density = new_density
+
+    @property
+    def dtype(self):
+        return self._dtype
+
+    @dtype.setter
+    def dtype(self, new_dtype):
+        self._dtype = np.dtype(new_dtype)
+
+    @property
+    def cpx_dtype(self):
+        return np.dtype(DTYPE_R2C[str(self.dtype)])
+
+    @property
+    def samples(self):
+        return self._samples
+
+    @samples.setter
+    def samples(self, new_samples):
+        self.update_samples(new_samples)
+
+    def update_samples(self, new_samples, *, unsafe=False):
+        self._samples = new_samples
+
+    @property
+    def n_samples(self):
+        return self._samples.shape[0]
+
+    @property
+    def norm_factor(self):
+        """``sqrt(prod(shape) * 2**ndim)`` (``base.py:837-840``)."""
+        return float(np.sqrt(np.prod(self.shape) * (2 ** len(self.shape))))
+
+    @property
+    def inv_norm_factor(self):
+        return 1.0 / self.norm_factor
+
+    # -------------------------------------------------------------- methods
+    def data_consistency(self, image_data, obs_data):
+        """Naive ``adj_op(op(x) - y)``; backends override with a fused version."""
+        return self.adj_op(self.op(image_data) - obs_data)
+
+    def gram_op(self, data, toeplitz: bool = False):
+        """``adj_op(op(x))``; the Toeplitz embedding is not provided by this backend."""
+        self.check_shape(image=data)
+        if toeplitz:
+            raise NotImplementedError("Toeplitz Gram operator is not implemented by hipnufft")
+        return self.adj_op(self.op(data))
+
+    def compute_density(self, method=None):
+        """Set the density compensation (array, True/'pipe', dict, name or callable)."""
+        from .density import get_density
+
+        kwargs = {}
+        if method is None or method is False:
+            self._density = None
+            return
+        if isinstance(method, (np.ndarray, torch.Tensor)):
+            self.density = method
+            return
+        if method is True or method == "pipe":
+            method_ = get_density("pipe")
+            kwargs["backend"] = self.backend
+        elif isinstance(method, dict) and isinstance(method.get("name"), str):
+            kwargs = {k: v for k, v in method.items() if k != "name"}
+            if method["name"] == "pipe" and "backend" not in kwargs:
+                kwargs["backend"] = self.backend
+            method_ = get_density(method["name"])
+        elif isinstance(method, str):
+            method_ = get_density(method)
+        elif callable(method):
+            method_ = method
+        else:
+            raise ValueError(f"Unknown density method: {method}")
+        self.density = method_(self.samples, self.shape, **kwargs)
+        if self._density_method is None:
+            self._density_method = partial(method_, **kwargs)
+
+    def compute_smaps(self, method=None):
+        """Set the sensitivity maps (array or callable(samples, shape, ...))."""
+        if method is None:
+            self.smaps = None
+            return
+        if isinstance(method, (np.ndarray, torch.Tensor)):
+            self.smaps = method.reshape(self.n_coils if method.shape[0] == self.n_coils
+                                        else method.shape[0], *self.shape)
+            return
+        if callable(method):
+            smaps = method(self.samples, self.shape, density=self.density, backend=self.backend)
+            self.smaps = smaps.reshape(-1, *self.shape)
+            return
+        raise ValueError(f"Unknown smaps method: {method}")
+
+    def get_lipschitz_cst(self, max_iter=10):
+        """Spectral radius of A^H A without coils (power method)."""
+        n_coils, n_batchs, smaps, sq = self.n_coils, self.n_batchs, self.smaps, self.squeeze_dims
+        self.smaps = None
+        self.n_coils = 1
+        self.n_batchs = 1
+        self.squeeze_dims = True
+        try:
+            lipschitz_cst, _ = power_method(max_iter, self)
+        finally:
+            self.n_coils = n_coils
+            self.n_batchs = n_batchs
+            self.smaps = smaps
+            self.squeeze_dims = sq
+        return lipschitz_cst
+
+    def pinv_solver(self, kspace_data, optim="lsqr", **kwargs):
+        """Least-squares reconstruction with a registered optimizer."""
+        from .optim import get_optimizer
+
+        return get_optimizer(optim)(operator=self, kspace_data=kspace_data, **kwargs)
+
+    def make_autograd(self, *, wrt_data=True, wrt_traj=False, paired_batch=False):
+        if not self.autograd_available:
+            raise ValueError("Backend does not support auto-differentiation.")
+        from .autodiff import MRINufftAutoGrad
+
+        return MRINufftAutoGrad(self, wrt_data=wrt_data, wrt_traj=wrt_traj,
+                                paired_batch=paired_batch)
+
+    @classmethod
+    def with_autograd(cls, wrt_data=True, wrt_traj=False, paired_batch=False, *args, **kwargs):
+        return cls(*args, **kwargs).make_autograd(wrt_data=wrt_data, wrt_traj=wrt_traj,
+                                                  paired_batch=paired_batch)
+
+    def make_linops(self, *, cupy: bool = False):
+        """scipy LinearOperator over flattened host arrays (``base.py:450-490``)."""
+        if cupy:
+            raise ValueError("cupy is not available")
+        from scipy.sparse.linalg import LinearOperator
+
+        ic = 1 if self.uses_sense else self.n_coils
+        linop = LinearOperator(
+            shape=(self.n_batchs * self.n_coils * self.n_samples,
+                   self.n_batchs * ic * int(np.prod(self.shape))),
+            matvec=lambda x: np.asarray(self.op(x.reshape(self.n_batchs, ic, *self.shape))).ravel(),
+            rmatvec=lambda x: np.asarray(
+                self.adj_op(x.reshape(self.n_batchs, self.n_coils, self.n_samples))).ravel(),
+            dtype=self.cpx_dtype,
+        )
+        linop._nufft = self
+        return linop
+
+    def __repr__(self):
+        return (f"{self.__class__.__name__}(\n  shape: {self.shape}\n  n_coils: {self.n_coils}\n"
+                f"  n_samples: {self.n_samples}\n  uses_sense: {self.uses_sense}\n)")
+
+
+def power_method(max_iter, operator, norm_func=None, x=None):
+    """Largest eigenvalue of A^H A (``base.py:1155-1221``).
+
+    Returns (norm, eigenvector).  Works on numpy arrays or torch tensors.
+    """
+
+    def AHA(v):
+        if callable(operator) and not isinstance(operator, FourierOperatorBase):
+            return operator(v)
+        return operator.adj_op(operator.op(v))
+
+    if norm_func is None:
+        def norm_func(v):
+            if isinstance(v, torch.Tensor):
+                return torch.linalg.vector_norm(v.reshape(-1))
+            return np.linalg.norm(v)
+
+    return_as_is = True
+    if x is None and isinstance(operator, FourierOperatorBase):
+        return_as_is = False
+        x = np.random.random(operator.shape).astype(operator.cpx_dtype)
+    elif x is None:
+        raise ValueError("If operator is not a FourierOperatorBase, an initial x should be provided.")
+    x_norm = norm_func(x)
+    x = x / x_norm
+    i = 0
+    for i in range(max_iter):  # noqa: B007
+        x_new = AHA(x)
+        x_new_norm = norm_func(x_new)
+        x_new = x_new / x_new_norm
+        if abs(float(x_norm) - float(x_new_norm)) < 1e-6:
+            break
+        x_norm = x_new_norm
+        x = x_new
+    if i == max_iter - 1:
+        warnings.warn("Lipschitz constant did not converge")
+    if return_as_is:
+        return x_new_norm, x_new
+    return float(x_new_norm), x_new
+
+
+class _ToggleGradPlanMixin:
+    """Swap in the trajectory-gradient plan (``base.py:1224-1238``)."""
+
+    @contextmanager
+    def grad_traj_plan(self):
+        self.toggle_grad_traj()
+        try:
+            yield
+        finally:
+            self.toggle_grad_traj()
+
+    def toggle_grad_traj(self):
+        if self.uses_sense:
+            self.smaps = self.smaps.conj()
+        self.raw_op.toggle_grad_traj()
+
+
+def array_kind(x):
+    return kind_of(x)
