@@ -53,7 +53,9 @@ typedef struct hipnufft_opts {
   int spread_group;     /* transforms accumulated together in one spread WG */
   int interp_group;     /* transforms gathered together in one interp WG    */
   int max_subproblem;   /* max points per interp work item (dense bins split) */
-  int deterministic;    /* reserved: 1 = fixed-order spreading               */
+  int deterministic;    /* reserved (spreading is always fixed-order)        */
+  int spread_max;       /* max list entries per spread work item (dense
+                           tiles are split and their partials summed)        */
 } hipnufft_opts;
 
 /* Fill *opts with defaults.  Replaces the cufinufft Plan **kwargs
@@ -162,6 +164,18 @@ int hipnufft_plan_info(const hipnufft_plan* plan, int64_t* nf, int* width,
 /* Bytes of device workspace the plan holds (fine grids, sorted points,
  * lists, tables).  Used for HBM-budget admission. */
 int hipnufft_workspace_bytes(const hipnufft_plan* plan, size_t* bytes);
+
+/* Per-stage kernel timing (no counterpart in the reference, whose only
+ * instrumentation is the nvtx_mark decorator, operators/gpu_utils.py:52-70).
+ * While enabled, HIP events are recorded on the plan stream around every
+ * kernel launch; hipnufft_get_timing synchronises, returns the summed
+ * milliseconds and launch counts per stage (HIPNUFFT_STAGE_* order) and
+ * resets.  Stages: rows_fwd, cols_fwd, slabs_fwd, interp, spread, slabs_adj,
+ * cols_adj, rows_adj, grad_traj. */
+#define HIPNUFFT_NUM_STAGES 9
+int hipnufft_set_timing(hipnufft_plan* plan, int enable);
+int hipnufft_get_timing(hipnufft_plan* plan, double* ms, int64_t* launches,
+                        int nstage);
 
 /* Change the stream of subsequent calls. */
 int hipnufft_set_stream(hipnufft_plan* plan, void* stream);

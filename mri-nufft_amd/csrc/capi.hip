@@ -21,6 +21,26 @@ int set_error(int code, const std::string& msg) {
   return code;
 }
 
+StageScope::StageScope(const hipnufft_plan_s* cp, int stage)
+    : p_(const_cast<hipnufft_plan_s*>(cp)) {
+  if (!p_->timing) return;
+  if (p_->ev_free.empty()) {
+    EventPair e{};
+    if (hipEventCreate(&e.start) != hipSuccess || hipEventCreate(&e.stop) != hipSuccess) return;
+    p_->ev_free.push_back(e);
+  }
+  ev_ = p_->ev_free.back();
+  p_->ev_free.pop_back();
+  ev_.stage = stage;
+  on_ = hipEventRecord(ev_.start, p_->stream) == hipSuccess;
+}
+
+StageScope::~StageScope() {
+  if (!on_) return;
+  (void)hipEventRecord(ev_.stop, p_->stream);
+  p_->ev_used.push_back(ev_);
+}
+
 namespace {
 
 // ----------------------------------------------------------------------------
@@ -149,6 +169,11 @@ int floor_pow2(int64_t x) {
 
 void free_plan(hipnufft_plan_s* p) {
   if (!p) return;
+  for (auto* v : {&p->ev_used, &p->ev_free})
+    for (auto& e : *v) {
+      (void)hipEventDestroy(e.start);
+      (void)hipEventDestroy(e.stop);
+    }
   for (int d = 0; d < 3; ++d) {
     (void)hipFree(p->dec[d]);
     (void)hipFree(p->fft[d].tw);
@@ -161,6 +186,13 @@ void free_plan(hipnufft_plan_s* p) {
   (void)hipFree(p->sub_begin);
   (void)hipFree(p->list_pts);
   (void)hipFree(p->list_start);
+  (void)hipFree(p->it_tile);
+  (void)hipFree(p->it_begin);
+  (void)hipFree(p->it_slot);
+  (void)hipFree(p->dn_tile);
+  (void)hipFree(p->dn_slot0);
+  (void)hipFree(p->dn_count);
+  (void)hipFree(p->part);
   (void)hipFree(p->grid);
   (void)hipFree(p->kwork);
   delete p;
@@ -285,6 +317,7 @@ int hipnufft_default_opts(hipnufft_opts* o) {
   o->interp_group = 0;
   o->max_subproblem = 0;
   o->deterministic = 0;
+  o->spread_max = 0;
   return 0;
 }
 
@@ -347,9 +380,10 @@ int hipnufft_plan_create(int type, int ndim, const int64_t* nmodes, int iflag, i
     free_plan(p);
     return set_error(HIPNUFFT_ERR_ARG, "too many tiles");
   }
-  p->spread_group = o.spread_group > 0 ? o.spread_group : (ndim == 3 ? 2 : 4);
+  p->spread_group = o.spread_group > 0 ? o.spread_group : 8;
   p->interp_group = o.interp_group > 0 ? o.interp_group : (ndim == 3 ? 1 : 4);
   p->max_sub = o.max_subproblem > 0 ? o.max_subproblem : 1024;
+  p->spread_max = o.spread_max > 0 ? o.spread_max : 4096;
 
   int rc = 0;
   // deconvolution tables (padded axes hold a single 1)
@@ -428,7 +462,13 @@ int hipnufft_sense_adj(hipnufft_plan* p, const void* ksp, const void* smaps, con
   if (p->sio) {
     if (s) return set_error(HIPNUFFT_ERR_ARG, "spread/interp-only plans take no smaps");
     if (scale != 1.0f) return set_error(HIPNUFFT_ERR_ARG, "spread/interp-only adjoint has no scale");
-    return launch_spread(p, y, density, TT, x, img_size(p));
+    for (int t0 = 0; t0 < TT; t0 += p->ntrans) {
+      const int Tc = std::min(p->ntrans, TT - t0);
+      if ((rc = launch_spread(p, y + (int64_t)t0 * p->M, density, Tc, x + (int64_t)t0 * img_size(p),
+                              img_size(p))))
+        return rc;
+    }
+    return 0;
   }
   const int T = chunk_of(p, C, s != nullptr);
   for (int t0 = 0; t0 < TT; t0 += T) {
@@ -555,7 +595,34 @@ int hipnufft_workspace_bytes(const hipnufft_plan* p, size_t* bytes) {
   size_t b = p->bytes;
   b += p->M_cap * (sizeof(int32_t) + p->ndim * (sizeof(int32_t) + sizeof(float)));
   b += (p->list_cap + 2 * p->sub_cap + 2 * (p->nbins + 1)) * sizeof(int32_t);
+  b += (3 * p->items_cap + 3 * p->dense_cap) * sizeof(int32_t) + p->part_cap * sizeof(float2);
   *bytes = b;
+  return 0;
+}
+
+int hipnufft_set_timing(hipnufft_plan* p, int enable) {
+  if (!p) return set_error(HIPNUFFT_ERR_ARG, "null plan");
+  p->timing = enable ? 1 : 0;
+  return 0;
+}
+
+int hipnufft_get_timing(hipnufft_plan* p, double* ms, int64_t* launches, int nstage) {
+  if (!p) return set_error(HIPNUFFT_ERR_ARG, "null plan");
+  HN_HIP(hipStreamSynchronize(p->stream));
+  for (int i = 0; i < nstage; ++i) {
+    if (ms) ms[i] = 0.0;
+    if (launches) launches[i] = 0;
+  }
+  for (auto& e : p->ev_used) {
+    float t = 0.f;
+    HN_HIP(hipEventElapsedTime(&t, e.start, e.stop));
+    if (e.stage < nstage) {
+      if (ms) ms[e.stage] += t;
+      if (launches) launches[e.stage] += 1;
+    }
+    p->ev_free.push_back(e);
+  }
+  p->ev_used.clear();
   return 0;
 }
 

@@ -404,7 +404,7 @@ int lg2(int x) {
 size_t fft_lds(const FFTAxis& ax) { return sizeof(float2) * (size_t)ax.n * (ax.L + 2); }
 
 int strided(const hipnufft_plan_s* p, int axis, int T, int in_M, int out_M, int outer_M,
-            int sign) {
+            int sign, int stage) {
   const FFTAxis& ax = p->fft[axis];
   StridedArgs a{};
   a.data = p->grid;
@@ -427,6 +427,7 @@ int strided(const hipnufft_plan_s* p, int axis, int T, int in_M, int out_M, int 
   a.outer_M = outer_M;
   a.nouter = outer_M ? outer_M : a.outer_nf;
   const dim3 grd((unsigned)((a.nx + ax.L - 1) / ax.L), (unsigned)a.nouter, (unsigned)T);
+  StageScope timer(p, stage);
   if (sign < 0)
     hipLaunchKernelGGL(k_fft_strided<-1>, grd, dim3(kThreads), fft_lds(ax), p->stream, a);
   else
@@ -472,14 +473,20 @@ int fwd_to_grid(const hipnufft_plan_s* p, const float2* img, const float2* smaps
   const int L = p->fft[2].L;
   const dim3 grd((unsigned)((rows + L - 1) / L), (unsigned)T);
   const size_t lds = fft_lds(p->fft[2]);
-  if (sign < 0)
-    hipLaunchKernelGGL(k_rows_fwd<-1>, grd, dim3(kThreads), lds, p->stream, a);
-  else
-    hipLaunchKernelGGL(k_rows_fwd<1>, grd, dim3(kThreads), lds, p->stream, a);
-  HN_LAUNCH_CHECK();
+  {
+    StageScope timer(p, kStageRowsFwd);
+    if (sign < 0)
+      hipLaunchKernelGGL(k_rows_fwd<-1>, grd, dim3(kThreads), lds, p->stream, a);
+    else
+      hipLaunchKernelGGL(k_rows_fwd<1>, grd, dim3(kThreads), lds, p->stream, a);
+    HN_LAUNCH_CHECK();
+  }
   int rc;
-  if (p->ndim >= 2 && (rc = strided(p, 1, T, (int)p->N[1], 0, (int)p->N[0], sign))) return rc;
-  if (p->ndim == 3 && (rc = strided(p, 0, T, (int)p->N[0], 0, 0, sign))) return rc;
+  if (p->ndim >= 2 &&
+      (rc = strided(p, 1, T, (int)p->N[1], 0, (int)p->N[0], sign, kStageColsFwd)))
+    return rc;
+  if (p->ndim == 3 && (rc = strided(p, 0, T, (int)p->N[0], 0, 0, sign, kStageSlabsFwd)))
+    return rc;
   return 0;
 }
 
@@ -487,8 +494,11 @@ int grid_to_img(const hipnufft_plan_s* p, float2* img, const float2* smaps, int 
                 float scale, int sign) {
   if (T <= 0) return 0;
   int rc;
-  if (p->ndim == 3 && (rc = strided(p, 0, T, 0, (int)p->N[0], 0, sign))) return rc;
-  if (p->ndim >= 2 && (rc = strided(p, 1, T, 0, (int)p->N[1], (int)p->N[0], sign))) return rc;
+  if (p->ndim == 3 && (rc = strided(p, 0, T, 0, (int)p->N[0], 0, sign, kStageSlabsAdj)))
+    return rc;
+  if (p->ndim >= 2 &&
+      (rc = strided(p, 1, T, 0, (int)p->N[1], (int)p->N[0], sign, kStageColsAdj)))
+    return rc;
   RowsArgs a = rows_args(p);
   a.img_out = img;
   a.smaps = smaps;
@@ -501,6 +511,7 @@ int grid_to_img(const hipnufft_plan_s* p, float2* img, const float2* smaps, int 
   const int nimg = smaps ? ((t0 + T - 1) / C - t0 / C + 1) : T;
   const dim3 grd((unsigned)((rows + L - 1) / L), (unsigned)nimg);
   const size_t lds = fft_lds(p->fft[2]);
+  StageScope timer(p, kStageRowsAdj);
   if (sign < 0)
     hipLaunchKernelGGL(k_rows_adj<-1>, grd, dim3(kThreads), lds, p->stream, a);
   else

@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "hipnufft.h"
 
@@ -57,6 +58,25 @@ struct PtsGeom {
   const int32_t* idx = nullptr;                          // sorted -> original
 };
 
+// Kernel stages timed by the optional per-plan instrumentation.
+enum Stage {
+  kStageRowsFwd = 0,   // deconvolve / expand / pad / FFT along the contiguous axis
+  kStageColsFwd,       // FFT along axis 1 (forward chain)
+  kStageSlabsFwd,      // FFT along axis 0 (forward chain, 3D)
+  kStageInterp,        // interpolation (type 2)
+  kStageSpread,        // spreading (type 1)
+  kStageSlabsAdj,      // FFT along axis 0 (adjoint chain, 3D)
+  kStageColsAdj,       // FFT along axis 1 (adjoint chain)
+  kStageRowsAdj,       // FFT / crop / deconvolve / coil combine (contiguous axis)
+  kStageGradTraj,      // kernel-derivative interpolation
+  kNumStages
+};
+
+struct EventPair {
+  int stage;
+  hipEvent_t start, stop;
+};
+
 }  // namespace hn
 
 struct hipnufft_plan_s {
@@ -92,6 +112,25 @@ struct hipnufft_plan_s {
   int64_t nlist = 0;
   int64_t M_cap = 0, list_cap = 0, sub_cap = 0;
 
+  // spread work items: dense tiles are split into items of at most
+  // spread_max entries whose partial tiles are summed in a fixed order
+  int spread_max = 4096;
+  int32_t* it_tile = nullptr;   // per item
+  int32_t* it_begin = nullptr;
+  int32_t* it_slot = nullptr;   // -1: the item writes the grid directly
+  int64_t nitems = 0, items_cap = 0;
+  int32_t* dn_tile = nullptr;   // per dense tile
+  int32_t* dn_slot0 = nullptr;
+  int32_t* dn_count = nullptr;
+  int64_t ndense = 0, dense_cap = 0;
+  int64_t nslots = 0;
+  float2* part = nullptr;       // nslots * ntrans * tile cells
+  int64_t part_cap = 0;
+
+  // optional per-stage timing (hipnufft_set_timing)
+  int timing = 0;
+  std::vector<hn::EventPair> ev_used, ev_free;
+
   // workspace
   float2* grid = nullptr;   // ntrans * prod(nf)
   float2* kwork = nullptr;  // ntrans * M (data-consistency residual)
@@ -100,6 +139,19 @@ struct hipnufft_plan_s {
 };
 
 namespace hn {
+
+// Records HIP events around one kernel launch on the plan stream when the
+// plan's timing is enabled (no-op otherwise).
+class StageScope {
+ public:
+  StageScope(const hipnufft_plan_s* p, int stage);
+  ~StageScope();
+
+ private:
+  hipnufft_plan_s* p_;
+  EventPair ev_{};
+  bool on_ = false;
+};
 
 int64_t grid_size(const hipnufft_plan_s* p);
 PtsGeom pts_geom(const hipnufft_plan_s* p);

@@ -272,6 +272,45 @@ __global__ __launch_bounds__(kThreads) void k_sub_emit(const int32_t* bs, const 
   }
 }
 
+// spread work items per tile: 1 for light tiles (also empty ones: every tile
+// is written), ceil(E / S) for dense ones; dense tiles also get a flag
+__global__ __launch_bounds__(kThreads) void k_item_count(const int32_t* ls, int64_t nbins, int S,
+                                                         int32_t* cnt, int32_t* dense_cnt,
+                                                         int32_t* dense_flag) {
+  const int64_t b = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (b >= nbins) return;
+  const int e = ls[b + 1] - ls[b];
+  const int k = e <= S ? 1 : (e + S - 1) / S;
+  cnt[b] = k;
+  dense_cnt[b] = k > 1 ? k : 0;
+  dense_flag[b] = k > 1 ? 1 : 0;
+}
+
+__global__ __launch_bounds__(kThreads) void k_item_emit(const int32_t* ls, int64_t nbins, int S,
+                                                        const int32_t* item_off,
+                                                        const int32_t* slot_off,
+                                                        const int32_t* dense_off,
+                                                        int32_t* it_tile, int32_t* it_begin,
+                                                        int32_t* it_slot, int32_t* dn_tile,
+                                                        int32_t* dn_slot0, int32_t* dn_count) {
+  const int64_t b = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (b >= nbins) return;
+  const int e = ls[b + 1] - ls[b];
+  const int k = e <= S ? 1 : (e + S - 1) / S;
+  for (int s = 0; s < k; ++s) {
+    const int i = item_off[b] + s;
+    it_tile[i] = (int32_t)b;
+    it_begin[i] = ls[b] + s * S;
+    it_slot[i] = k > 1 ? slot_off[b] + s : -1;
+  }
+  if (k > 1) {
+    const int j = dense_off[b];
+    dn_tile[j] = (int32_t)b;
+    dn_slot0[j] = slot_off[b];
+    dn_count[j] = k;
+  }
+}
+
 // distinct tiles touched along one axis, in order of first appearance
 __device__ inline int axis_tiles(int c0, int w, int nf, int tile, int* t) {
   int n = 0;
@@ -383,6 +422,62 @@ int scan_exclusive(const int32_t* in, int32_t* out, int64_t n, hipStream_t s, in
   return 0;
 }
 
+// Spread work items from the list offsets (see k_item_count) and the
+// partial-tile scratch for the dense tiles.
+static int build_spread_items(hipnufft_plan_s* p) {
+  hipStream_t s = p->stream;
+  const int64_t nbins = p->nbins;
+  const int S = p->spread_max;
+  int rc;
+  int32_t *cnt = nullptr, *dcnt = nullptr, *dflag = nullptr;
+  HN_HIP(hipMallocAsync((void**)&cnt, sizeof(int32_t) * nbins, s));
+  HN_HIP(hipMallocAsync((void**)&dcnt, sizeof(int32_t) * nbins, s));
+  HN_HIP(hipMallocAsync((void**)&dflag, sizeof(int32_t) * nbins, s));
+  const unsigned g = (unsigned)((nbins + kThreads - 1) / kThreads);
+  hipLaunchKernelGGL(k_item_count, dim3(g), dim3(kThreads), 0, s, p->list_start, nbins, S, cnt,
+                     dcnt, dflag);
+  HN_LAUNCH_CHECK();
+  int64_t nitems = 0, nslots = 0, ndense = 0;
+  if ((rc = scan_exclusive(cnt, cnt, nbins, s, &nitems))) return rc;
+  if ((rc = scan_exclusive(dcnt, dcnt, nbins, s, &nslots))) return rc;
+  if ((rc = scan_exclusive(dflag, dflag, nbins, s, &ndense))) return rc;
+  if (nitems > p->items_cap || !p->it_tile) {
+    int64_t c0 = 0, c1 = 0, c2 = 0;
+    if ((rc = grow(&p->it_tile, &c0, nitems))) return rc;
+    if ((rc = grow(&p->it_begin, &c1, nitems))) return rc;
+    if ((rc = grow(&p->it_slot, &c2, nitems))) return rc;
+    p->items_cap = c0;
+  }
+  if (ndense > p->dense_cap || !p->dn_tile) {
+    int64_t c0 = 0, c1 = 0, c2 = 0;
+    if ((rc = grow(&p->dn_tile, &c0, ndense))) return rc;
+    if ((rc = grow(&p->dn_slot0, &c1, ndense))) return rc;
+    if ((rc = grow(&p->dn_count, &c2, ndense))) return rc;
+    p->dense_cap = c0;
+  }
+  int64_t tcells = 1;
+  for (int d = 0; d < 3; ++d) tcells *= p->tile[d];
+  const int64_t part_need = nslots * (int64_t)p->ntrans * tcells;
+  if (part_need > p->part_cap) {
+    if (p->part) HN_HIP(hipFree(p->part));
+    p->part = nullptr;
+    if (hipMalloc((void**)&p->part, sizeof(float2) * part_need) != hipSuccess)
+      return set_error(HIPNUFFT_ERR_ALLOC, "cannot allocate the spreading scratch");
+    p->part_cap = part_need;
+  }
+  hipLaunchKernelGGL(k_item_emit, dim3(g), dim3(kThreads), 0, s, p->list_start, nbins, S, cnt,
+                     dcnt, dflag, p->it_tile, p->it_begin, p->it_slot, p->dn_tile, p->dn_slot0,
+                     p->dn_count);
+  HN_LAUNCH_CHECK();
+  p->nitems = nitems;
+  p->nslots = nslots;
+  p->ndense = ndense;
+  HN_HIP(hipFreeAsync(cnt, s));
+  HN_HIP(hipFreeAsync(dcnt, s));
+  HN_HIP(hipFreeAsync(dflag, s));
+  return 0;
+}
+
 int build_points(hipnufft_plan_s* p, int64_t M, const float* const* xd) {
   const int D = p->ndim;
   const int o = 3 - D;  // first real axis in the padded arrays
@@ -422,6 +517,8 @@ int build_points(hipnufft_plan_s* p, int64_t M, const float* const* xd) {
     HN_HIP(hipMemsetAsync(p->list_start, 0, sizeof(int32_t) * (nbins + 1), s));
     p->nsub = 0;
     p->nlist = 0;
+    int rc0 = build_spread_items(p);  // empty tiles still write zeros
+    if (rc0) return rc0;
     HN_HIP(hipStreamSynchronize(s));
     return 0;
   }
@@ -553,6 +650,8 @@ int build_points(hipnufft_plan_s* p, int64_t M, const float* const* xd) {
     HN_HIP(hipFreeAsync(lk1, s));
     HN_HIP(hipFreeAsync(lv1, s));
   }
+
+  if ((rc = build_spread_items(p))) return rc;
 
   HN_HIP(hipFreeAsync(ka, s));
   HN_HIP(hipFreeAsync(kb, s));

@@ -43,6 +43,15 @@ struct SpreadArgs {
   PtsGeom g;
   const int32_t* list_pts;
   const int32_t* list_start;
+  const int32_t* it_tile;   // work items (dense tiles split)
+  const int32_t* it_begin;
+  const int32_t* it_slot;
+  const int32_t* dn_tile;   // dense tiles: partial slots to sum
+  const int32_t* dn_slot0;
+  const int32_t* dn_count;
+  float2* part;             // [slot][ntrans][tile cells]
+  int ntrans;
+  int smax;
   const float2* in;
   int64_t in_stride;
   const float* density;
@@ -80,17 +89,141 @@ __device__ __forceinline__ void tile_coords(int key, const int* nt, int* tc) {
 }
 
 // ---------------------------------------------------------------------------
-// spreading
+// spreading: owner-computes gather, no atomics
 // ---------------------------------------------------------------------------
-template <int D, int W>
+// Each thread owns up to kCellsPerThread cells of the tile and keeps their
+// sums for the G transforms in registers.  The tile's list entries are staged
+// through LDS kSpreadChunk at a time (footprint start relative to the tile,
+// the w weights per axis, the G strengths); every thread then walks the
+// staged points and adds the ones whose footprint covers its cells.  Points
+// read from LDS are wave-uniform (broadcast), and each cell's sum is formed
+// in list order, so the result is deterministic bit for bit.  Thousands of
+// radial spokes crossing the k-space centre cost no contention.
+constexpr int kCellsPerThread = 4;  // tiles hold at most 1024 cells
+constexpr int kSpreadChunk = kThreads;
+
+template <int D, int G>
 __global__ __launch_bounds__(kThreads) void k_spread(SpreadArgs a) {
-  extern __shared__ float2 lds[];
-  float* ldsf = reinterpret_cast<float*>(lds);
-  const int tile = blockIdx.x;
-  const int t0 = blockIdx.y * a.G;
-  const int G = min(a.G, a.T - t0);
+  extern __shared__ float smem[];
+  const int W = a.g.w;
+  const int PW = D * W + D + 2 * G;  // floats per staged point
+  const int item = blockIdx.x;
+  const int tile = a.it_tile[item];
+  const int slot = a.it_slot[item];
+  const int t0 = blockIdx.y * G;
+  const int Gv = min(G, a.T - t0);
   const float inv_hw = 2.0f / W, beta = a.g.beta;
 
+  int tc[3], org[3], ext[3], td[3], nf[3];
+  tile_coords<D>(tile, a.g.nt, tc);
+  int TC = 1;
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    td[d] = a.g.tile[d];
+    nf[d] = a.g.nf[d];
+    org[d] = tc[d] * td[d];
+    ext[d] = min(td[d], nf[d] - org[d]);
+    TC *= td[d];
+  }
+  int cl[kCellsPerThread][3];
+  bool own[kCellsPerThread];
+#pragma unroll
+  for (int k = 0; k < kCellsPerThread; ++k) {
+    int r = threadIdx.x + k * kThreads;
+    own[k] = r < TC;
+#pragma unroll
+    for (int d = D - 1; d >= 0; --d) {
+      cl[k][d] = r % td[d];
+      r /= td[d];
+      own[k] = own[k] && cl[k][d] < ext[d];
+    }
+  }
+  float2 acc[kCellsPerThread][G];
+#pragma unroll
+  for (int k = 0; k < kCellsPerThread; ++k)
+#pragma unroll
+    for (int g = 0; g < G; ++g) acc[k][g] = make_float2(0.f, 0.f);
+
+  const int e0 = a.it_begin[item];
+  const int e1 = min(e0 + a.smax, a.list_start[tile + 1]);
+  for (int eb = e0; eb < e1; eb += kSpreadChunk) {
+    const int n = min(kSpreadChunk, e1 - eb);
+    if ((int)threadIdx.x < n) {
+      float* s = smem + threadIdx.x * PW;
+      const int p = a.list_pts[eb + threadIdx.x];
+      const int idx = a.g.idx[p];
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        const float off = a.g.off[d][p];
+        reinterpret_cast<int*>(s)[d] = a.g.cell[d][p] - org[d];  // raw start, mod nf
+        for (int j = 0; j < W; ++j) s[D + d * W + j] = es_eval(off - (float)j, inv_hw, beta);
+      }
+      const float dens = a.density ? a.density[idx] : 1.0f;
+      float* sv = s + D + D * W;
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        float2 v = make_float2(0.f, 0.f);
+        if (g < Gv) v = a.in[(int64_t)(t0 + g) * a.in_stride + idx];
+        sv[2 * g] = v.x * dens;
+        sv[2 * g + 1] = v.y * dens;
+      }
+    }
+    __syncthreads();
+    for (int q = 0; q < n; ++q) {
+      const float* s = smem + q * PW;
+      int ls[3];
+#pragma unroll
+      for (int d = 0; d < D; ++d) ls[d] = reinterpret_cast<const int*>(s)[d];
+      const float* sv = s + D + D * W;
+#pragma unroll
+      for (int k = 0; k < kCellsPerThread; ++k) {
+        if (!own[k]) continue;
+        float wgt = 1.0f;
+        bool hit = true;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          int da = cl[k][d] - ls[d];
+          if (da < 0) da += nf[d];
+          if (da >= nf[d]) da -= nf[d];
+          hit = hit && da < W;
+          wgt *= s[D + d * W + (hit ? da : 0)];  // index stays inside the record
+        }
+        if (!hit) continue;
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+          acc[k][g].x += wgt * sv[2 * g];
+          acc[k][g].y += wgt * sv[2 * g + 1];
+        }
+      }
+    }
+    __syncthreads();
+  }
+
+#pragma unroll
+  for (int k = 0; k < kCellsPerThread; ++k) {
+    if (!own[k]) continue;
+    if (slot >= 0) {  // partial tile of a dense tile: summed by k_spread_reduce
+      const int r = threadIdx.x + k * kThreads;
+#pragma unroll
+      for (int g = 0; g < G; ++g)
+        if (g < Gv) a.part[((int64_t)slot * a.ntrans + t0 + g) * TC + r] = acc[k][g];
+      continue;
+    }
+    int64_t gi = 0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) gi = gi * nf[d] + org[d] + cl[k][d];
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+      if (g < Gv) a.grid[(int64_t)(t0 + g) * a.grid_stride + gi] = acc[k][g];
+  }
+}
+
+// Sum the partial tiles of each dense tile in slot order (deterministic).
+template <int D>
+__global__ __launch_bounds__(kThreads) void k_spread_reduce(SpreadArgs a) {
+  const int j = blockIdx.x;
+  const int t = blockIdx.y;
+  const int tile = a.dn_tile[j], slot0 = a.dn_slot0[j], cnt = a.dn_count[j];
   int tc[3], org[3], ext[3], td[3];
   tile_coords<D>(tile, a.g.nt, tc);
   int TC = 1;
@@ -101,104 +234,29 @@ __global__ __launch_bounds__(kThreads) void k_spread(SpreadArgs a) {
     ext[d] = min(td[d], a.g.nf[d] - org[d]);
     TC *= td[d];
   }
-  for (int c = threadIdx.x; c < G * TC; c += kThreads) lds[c] = make_float2(0.f, 0.f);
-  __syncthreads();
-
-  const int e0 = a.list_start[tile], e1 = a.list_start[tile + 1];
-  for (int e = e0 + threadIdx.x; e < e1; e += kThreads) {
-    const int p = a.list_pts[e];
-    const int idx = a.g.idx[p];
-    int li[D][W];
-    float wt[D][W];
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-      const int c0 = a.g.cell[d][p];
-      const float off = a.g.off[d][p];
-#pragma unroll
-      for (int j = 0; j < W; ++j) {
-        int c = c0 + j;
-        if (c >= a.g.nf[d]) c -= a.g.nf[d];
-        const int l = c - org[d];
-        li[d][j] = ((unsigned)l < (unsigned)ext[d]) ? l : -1;
-        wt[d][j] = es_eval(off - (float)j, inv_hw, beta);
-      }
-    }
-    const float dens = a.density ? a.density[idx] : 1.0f;
-    for (int gg = 0; gg < G; ++gg) {
-      float2 v = a.in[(int64_t)(t0 + gg) * a.in_stride + idx];
-      v.x *= dens;
-      v.y *= dens;
-      float* base = ldsf + 2 * gg * TC;
-      if constexpr (D == 1) {
-#pragma unroll
-        for (int j = 0; j < W; ++j)
-          if (li[0][j] >= 0) {
-            atomicAdd(base + 2 * li[0][j], wt[0][j] * v.x);
-            atomicAdd(base + 2 * li[0][j] + 1, wt[0][j] * v.y);
-          }
-      } else if constexpr (D == 2) {
-#pragma unroll
-        for (int j0 = 0; j0 < W; ++j0) {
-          if (li[0][j0] < 0) continue;
-          const float vx = wt[0][j0] * v.x, vy = wt[0][j0] * v.y;
-          float* row = base + 2 * li[0][j0] * td[1];
-#pragma unroll
-          for (int j1 = 0; j1 < W; ++j1)
-            if (li[1][j1] >= 0) {
-              atomicAdd(row + 2 * li[1][j1], wt[1][j1] * vx);
-              atomicAdd(row + 2 * li[1][j1] + 1, wt[1][j1] * vy);
-            }
-        }
-      } else {
-        // axis 0 is looped (not unrolled): its weight is recomputed per tap
-        // so no register array is indexed with a runtime index
-        const int c00 = a.g.cell[0][p];
-        const float off0 = a.g.off[0][p];
-#pragma unroll 1
-        for (int j0 = 0; j0 < W; ++j0) {
-          int c = c00 + j0;
-          if (c >= a.g.nf[0]) c -= a.g.nf[0];
-          const int l0 = c - org[0];
-          if ((unsigned)l0 >= (unsigned)ext[0]) continue;
-          const float w0 = es_eval(off0 - (float)j0, inv_hw, beta);
-          const float ux = w0 * v.x, uy = w0 * v.y;
-#pragma unroll
-          for (int j1 = 0; j1 < W; ++j1) {
-            if (li[1][j1] < 0) continue;
-            const float vx = wt[1][j1] * ux, vy = wt[1][j1] * uy;
-            float* row = base + 2 * (l0 * td[1] + li[1][j1]) * td[2];
-#pragma unroll
-            for (int j2 = 0; j2 < W; ++j2)
-              if (li[2][j2] >= 0) {
-                atomicAdd(row + 2 * li[2][j2], wt[2][j2] * vx);
-                atomicAdd(row + 2 * li[2][j2] + 1, wt[2][j2] * vy);
-              }
-          }
-        }
-      }
-    }
-  }
-  __syncthreads();
-
-  for (int c = threadIdx.x; c < G * TC; c += kThreads) {
-    const int gg = c / TC;
-    int r = c - gg * TC;
-    int l[3];
+  for (int r = threadIdx.x; r < TC; r += kThreads) {
+    int rr = r, l[3];
+    bool ok = true;
 #pragma unroll
     for (int d = D - 1; d >= 0; --d) {
-      l[d] = r % td[d];
-      r /= td[d];
+      l[d] = rr % td[d];
+      rr /= td[d];
+      ok = ok && l[d] < ext[d];
     }
-    bool ok = true;
+    if (!ok) continue;
+    float2 sum = make_float2(0.f, 0.f);
+    for (int q = 0; q < cnt; ++q) {
+      const float2 v = a.part[((int64_t)(slot0 + q) * a.ntrans + t) * TC + r];
+      sum.x += v.x;
+      sum.y += v.y;
+    }
     int64_t gi = 0;
 #pragma unroll
-    for (int d = 0; d < D; ++d) {
-      ok &= l[d] < ext[d];
-      gi = gi * a.g.nf[d] + org[d] + l[d];
-    }
-    if (ok) a.grid[(int64_t)(t0 + gg) * a.grid_stride + gi] = lds[c];
+    for (int d = 0; d < D; ++d) gi = gi * a.g.nf[d] + org[d] + l[d];
+    a.grid[(int64_t)t * a.grid_stride + gi] = sum;
   }
 }
+
 
 // ---------------------------------------------------------------------------
 // interpolation (and the kernel-derivative gradient)
@@ -500,22 +558,54 @@ int launch_spread(const hipnufft_plan_s* p, const float2* in, const float* densi
   a.g = pts_geom(p);
   a.list_pts = p->list_pts;
   a.list_start = p->list_start;
+  a.it_tile = p->it_tile;
+  a.it_begin = p->it_begin;
+  a.it_slot = p->it_slot;
+  a.dn_tile = p->dn_tile;
+  a.dn_slot0 = p->dn_slot0;
+  a.dn_count = p->dn_count;
+  a.part = p->part;
+  a.ntrans = p->ntrans;
+  a.smax = p->spread_max;
   a.in = in;
   a.in_stride = p->M;
   a.density = density;
   a.grid = grid;
   a.grid_stride = grid_stride;
   a.T = T;
-  const size_t cell_bytes = sizeof(float2) * (size_t)tile_cells(p, 0);
+  if (T > p->ntrans) return set_error(HIPNUFFT_ERR_ARG, "spread chunk larger than ntrans");
+  if (tile_cells(p, 0) > kCellsPerThread * kThreads)
+    return set_error(HIPNUFFT_ERR_ARG, "spread tile larger than 1024 cells");
   int G = p->spread_group;
-  while (G > 1 && cell_bytes * G > kMaxLds) --G;
+  G = G >= 8 ? 8 : G >= 4 ? 4 : G >= 2 ? 2 : 1;
+  while (G > 1 && G / 2 >= T) G /= 2;
+  auto lds_of = [&](int g) {
+    return sizeof(float) * (size_t)kSpreadChunk * (p->ndim * p->w + p->ndim + 2 * g);
+  };
+  while (G > 1 && lds_of(G) > kMaxLds) G /= 2;
   a.G = G;
-  const size_t lds = cell_bytes * G;
-  const dim3 grd((unsigned)p->nbins, (unsigned)((T + G - 1) / G));
-  HN_DIM_SWITCH(p->ndim, HN_WIDTH_SWITCH(p->w, {
-    hipLaunchKernelGGL((k_spread<DD, WW>), grd, dim3(kThreads), lds, p->stream, a);
-  }));
+  const size_t lds = lds_of(G);
+  const dim3 grd((unsigned)p->nitems, (unsigned)((T + G - 1) / G));
+  StageScope timer(p, kStageSpread);
+#define HN_SPREAD(DD)                                                                        \
+  switch (G) {                                                                               \
+    case 8: hipLaunchKernelGGL((k_spread<DD, 8>), grd, dim3(kThreads), lds, p->stream, a); break; \
+    case 4: hipLaunchKernelGGL((k_spread<DD, 4>), grd, dim3(kThreads), lds, p->stream, a); break; \
+    case 2: hipLaunchKernelGGL((k_spread<DD, 2>), grd, dim3(kThreads), lds, p->stream, a); break; \
+    default: hipLaunchKernelGGL((k_spread<DD, 1>), grd, dim3(kThreads), lds, p->stream, a); break; \
+  }
+  if (p->ndim == 1) { HN_SPREAD(1) }
+  else if (p->ndim == 2) { HN_SPREAD(2) }
+  else { HN_SPREAD(3) }
+#undef HN_SPREAD
   HN_LAUNCH_CHECK();
+  if (p->ndense > 0) {
+    const dim3 rg((unsigned)p->ndense, (unsigned)T);
+    if (p->ndim == 1) hipLaunchKernelGGL(k_spread_reduce<1>, rg, dim3(kThreads), 0, p->stream, a);
+    if (p->ndim == 2) hipLaunchKernelGGL(k_spread_reduce<2>, rg, dim3(kThreads), 0, p->stream, a);
+    if (p->ndim == 3) hipLaunchKernelGGL(k_spread_reduce<3>, rg, dim3(kThreads), 0, p->stream, a);
+    HN_LAUNCH_CHECK();
+  }
   return 0;
 }
 
@@ -544,6 +634,7 @@ int launch_interp(const hipnufft_plan_s* p, const float2* grid, int64_t grid_str
   a.G = G;
   const size_t lds = cell_bytes * G;
   const dim3 grd((unsigned)p->nsub, (unsigned)((T + G - 1) / G));
+  StageScope timer(p, kStageInterp);
   HN_DIM_SWITCH(p->ndim, HN_WIDTH_SWITCH(p->w, {
     hipLaunchKernelGGL((k_interp<DD, WW>), grd, dim3(kThreads), lds, p->stream, a);
   }));
@@ -578,6 +669,7 @@ int launch_grad_traj(const hipnufft_plan_s* p, const float2* grid, int64_t grid_
     a.obs = dy + (int64_t)t0 * p->M;
     a.T = Tg;
     const size_t lds = cell_bytes * Tg;
+    StageScope timer(p, kStageGradTraj);
     HN_DIM_SWITCH(p->ndim, HN_WIDTH_SWITCH(p->w, {
       hipLaunchKernelGGL((k_grad_traj<DD, WW>), dim3((unsigned)p->nsub), dim3(kThreads), lds,
                          p->stream, a);

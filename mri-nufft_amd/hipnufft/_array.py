@@ -33,7 +33,9 @@ def kind_of(x):
 def to_device(x, dtype, device):
     """Contiguous tensor of ``dtype`` on ``device`` (zero copy when possible)."""
     if isinstance(x, torch.Tensor):
-        t = x.detach()
+        # lazy conj / neg views keep the original memory: materialise them,
+        # the C ABI reads raw device memory
+        t = x.detach().resolve_conj().resolve_neg()
     else:
         t = torch.from_numpy(np.ascontiguousarray(np.asarray(x)))
     if t.dtype != dtype and not t.is_complex() and dtype in (torch.complex64, torch.complex128):

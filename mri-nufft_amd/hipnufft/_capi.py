@@ -21,6 +21,9 @@ ERR_HIP = -2
 ERR_NOPTS = -3
 ERR_ALLOC = -4
 
+STAGES = ("rows_fwd", "cols_fwd", "slabs_fwd", "interp", "spread", "slabs_adj", "cols_adj",
+          "rows_adj", "grad_traj")
+
 _lock = threading.Lock()
 _lib = None
 
@@ -40,6 +43,7 @@ class Opts(ctypes.Structure):
         ("interp_group", ctypes.c_int),
         ("max_subproblem", ctypes.c_int),
         ("deterministic", ctypes.c_int),
+        ("spread_max", ctypes.c_int),
     ]
 
 
@@ -66,6 +70,8 @@ _SIGNATURES = {
                            ctypes.POINTER(_I64)],
     "hipnufft_workspace_bytes": [_P, ctypes.POINTER(ctypes.c_size_t)],
     "hipnufft_set_stream": [_P, _P],
+    "hipnufft_set_timing": [_P, _I],
+    "hipnufft_get_timing": [_P, _P, _P, _I],
     "hipnufft_plan_destroy": [_P],
 }
 
@@ -105,8 +111,16 @@ def check(rc: int, what: str = ""):
 
 
 def ptr(t) -> int | None:
-    """Device pointer of a tensor (None for None)."""
-    return None if t is None else t.data_ptr()
+    """Device pointer of a tensor (None for None).
+
+    The library reads raw memory, so lazy conj / neg views and strided
+    tensors are refused rather than silently read as their storage.
+    """
+    if t is None:
+        return None
+    if t.is_conj() or t.is_neg() or not t.is_contiguous():
+        raise ValueError("libhipnufft needs contiguous tensors without lazy conj/neg views")
+    return t.data_ptr()
 
 
 def current_stream() -> int:
@@ -123,7 +137,7 @@ class RawPlan:
 
     def __init__(self, shape, n_trans=1, eps=1e-6, isign=-1, upsampfac=2.0,
                  spreadinterponly=False, tile=None, spread_group=0,
-                 interp_group=0, max_subproblem=0, stream=None):
+                 interp_group=0, max_subproblem=0, spread_max=0, stream=None):
         L = lib()
         self.shape = tuple(int(s) for s in shape)
         self.ndim = len(self.shape)
@@ -140,6 +154,7 @@ class RawPlan:
         o.spread_group = int(spread_group)
         o.interp_group = int(interp_group)
         o.max_subproblem = int(max_subproblem)
+        o.spread_max = int(spread_max)
         self.opts = o
         nm = (_I64 * self.ndim)(*self.shape)
         h = _P()
@@ -181,6 +196,17 @@ class RawPlan:
         check(lib().hipnufft_plan_info(self._h, nf, ctypes.byref(w), ctypes.byref(beta), tile,
                                        ctypes.byref(M)))
         return dict(nf=tuple(nf), w=w.value, beta=beta.value, tile=tuple(tile), M=M.value)
+
+    def set_timing(self, enable=True):
+        check(lib().hipnufft_set_timing(self._h, 1 if enable else 0))
+
+    def get_timing(self):
+        """{stage: (total_ms, launches)} since the last read (synchronises)."""
+        n = len(STAGES)
+        ms = (ctypes.c_double * n)()
+        cnt = (_I64 * n)()
+        check(lib().hipnufft_get_timing(self._h, ms, cnt, n))
+        return {s: (ms[i], cnt[i]) for i, s in enumerate(STAGES)}
 
     def workspace_bytes(self):
         b = ctypes.c_size_t()
@@ -251,8 +277,8 @@ def dot_c64(a, b, conj=True, out=None, ws=None):
     device tensors.  Returns a (2,) float64 device tensor [re, im] without a
     host synchronisation; ``ws`` is an optional (DOT_WS,) float64 workspace.
     """
-    a = a.contiguous()
-    b = b.contiguous()
+    a = a.resolve_conj().resolve_neg().contiguous()
+    b = b.resolve_conj().resolve_neg().contiguous()
     if a.numel() != b.numel():
         raise ValueError("dot_c64: size mismatch")
     if out is None:

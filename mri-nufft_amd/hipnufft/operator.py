@@ -102,6 +102,7 @@ class MRIHipNUFFT(FourierOperatorBase, _ToggleGradPlanMixin):
             spread_group=int(kwargs.pop("spread_group", 0)),
             interp_group=int(kwargs.pop("interp_group", 0)),
             max_subproblem=int(kwargs.pop("max_subproblem", 0)),
+            spread_max=int(kwargs.pop("spread_max", 0)),
         )
         for k in ("gpu_method", "gpu_sort", "gpu_kerevalmeth", "gpu_spreadinterponly"):
             kwargs.pop(k, None)
@@ -259,14 +260,15 @@ class MRIHipNUFFT(FourierOperatorBase, _ToggleGradPlanMixin):
     def _op(self, image, coeffs):
         """Raw type 2 of ``image`` (T, *shape) into ``coeffs`` (T, M): no norm, no smaps."""
         T = image.shape[0] if image.ndim > self.ndim else 1
-        x = image.reshape(T, *self.shape).contiguous()
+        x = image.reshape(T, *self.shape).resolve_conj().resolve_neg().contiguous()
         self.raw_op.plan.sense_op(x, None, coeffs, T, 1, 1.0)
         return coeffs
 
     def _adj_op(self, coeffs, image):
         """Raw type 1 of ``coeffs`` (T, M) into ``image`` (T, *shape): no norm, no density."""
         T = coeffs.reshape(-1, self.n_samples).shape[0]
-        self.raw_op.plan.sense_adj(coeffs.contiguous(), None, None, image, T, 1, 1.0)
+        self.raw_op.plan.sense_adj(coeffs.resolve_conj().resolve_neg().contiguous(), None, None,
+                                   image, T, 1, 1.0)
         return image
 
     # --------------------------------------------------------------- extras

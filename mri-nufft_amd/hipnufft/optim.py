@@ -128,7 +128,8 @@ def _cg_graph(operator, y, image, grad, velocity, L, damp, x0, max_iter, tol, sh
     bufs = dict(nrm=torch.empty(2, dtype=torch.float64, device=dev),
                 num=torch.empty(2, dtype=torch.float64, device=dev),
                 den=torch.empty(2, dtype=torch.float64, device=dev),
-                ws=torch.empty(_capi.DOT_WS, dtype=torch.float64, device=dev))
+                ws=torch.empty(_capi.DOT_WS, dtype=torch.float64, device=dev),
+                zero=torch.zeros((), dtype=torch.float64, device=dev))
     inv_l = 1.0 / L
     tol2 = tol * tol
 
@@ -148,8 +149,8 @@ def _cg_graph(operator, y, image, grad, velocity, L, damp, x0, max_iter, tol, sh
         b_re = (n_re * d_re + n_im * d_im) / dd
         b_im = (n_im * d_re - n_re * d_im) / dd
         pos = (b_re > 0) | ((b_re == 0) & (b_im > 0))
-        b_re = torch.where(pos, b_re, torch.zeros_like(b_re))
-        b_im = torch.where(pos, b_im, torch.zeros_like(b_im))
+        b_re = torch.where(pos, b_re, bufs["zero"])
+        b_im = torch.where(pos, b_im, bufs["zero"])
         beta = torch.complex(b_re, b_im).to(torch.complex64)
         vel_new = gn + beta * st["vel"]
         img_new = st["image"] - vel_new * inv_l

@@ -1,0 +1,163 @@
+"""Host-side checks that need no GPU.
+
+* the C-ABI library loads and exports every entry point declared in
+  include/hipnufft.h (no compute call is made);
+* registry semantics of get_operator / list_backends / check_backend
+  (reference tests/operators/test_bindings.py:7-30, operators/base.py:45-179);
+* proper_trajectory (reference _utils.py:58-100) and the method registries;
+* trajectory generators against their closed forms;
+* bench.py refuses more than one GPU.
+"""
+
+import os
+import re
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_symbols():
+    txt = open(os.path.join(ROOT, "include", "hipnufft.h")).read()
+    return sorted(set(re.findall(r"\b(hipnufft_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_header_symbols():
+    from hipnufft import _capi
+
+    lib = _capi.lib()
+    declared = header_symbols()
+    assert "hipnufft_plan_create" in declared and "hipnufft_sense_dc" in declared
+    for name in declared:
+        assert hasattr(lib, name), name
+    # the ctypes table covers exactly the declared entry points
+    assert sorted(_capi.EXPORTED) == declared
+
+
+def test_opts_struct_layout():
+    import ctypes
+
+    from hipnufft import _capi
+
+    o = _capi.Opts()
+    assert _capi.lib().hipnufft_default_opts(ctypes.byref(o)) == 0
+    assert o.upsampfac == 2.0 and list(o.tile) == [0, 0, 0] and o.spreadinterponly == 0
+
+
+def test_registry_semantics():
+    import hipnufft
+    from hipnufft.base import FourierOperatorBase
+
+    assert "hipnufft" in hipnufft.list_backends()
+    with pytest.raises(ValueError):
+        hipnufft.check_backend("nope")
+    with pytest.raises(ValueError):
+        hipnufft.get_operator("nope")
+    avail = hipnufft.check_backend("hipnufft")
+    if not avail:
+        with pytest.raises(ValueError, match="dependencies are not met"):
+            hipnufft.get_operator("hipnufft")
+        assert "hipnufft" not in hipnufft.list_backends(available_only=True)
+
+    class Fake(FourierOperatorBase):
+        backend = "fake-test"
+        available = False
+
+    assert FourierOperatorBase.interfaces["fake-test"] == (False, Fake)
+    with pytest.raises(RuntimeError):
+        Fake()
+    del FourierOperatorBase.interfaces["fake-test"]
+
+
+def test_method_registries():
+    from hipnufft import get_density, get_optimizer
+
+    assert callable(get_optimizer("cg")) and callable(get_optimizer("lsqr"))
+    assert callable(get_optimizer("lsmr")) and callable(get_density("pipe"))
+    with pytest.raises(ValueError, match="Unknown optimizer method"):
+        get_optimizer("sgd")
+
+
+def test_proper_trajectory():
+    import torch
+
+    from hipnufft._array import proper_trajectory
+
+    unit = np.array([[[0.25, -0.5], [0.1, 0.0]]], dtype=np.float32)
+    out = proper_trajectory(unit, "pi")
+    assert out.shape == (2, 2) and out.dtype == np.float32
+    np.testing.assert_allclose(out, unit.reshape(-1, 2) * 2 * np.pi, rtol=1e-6)
+    rad = np.array([[np.pi / 2, -3.0]])
+    np.testing.assert_allclose(proper_trajectory(rad, "pi"), rad)
+    np.testing.assert_allclose(proper_trajectory(rad, "unit"), rad / (2 * np.pi))
+    t = torch.tensor([[0.25, 0.5]])
+    np.testing.assert_allclose(proper_trajectory(t, "unit").numpy(), [[0.25, -0.5]])
+
+
+def test_radial_closed_form():
+    from hipnufft.trajectories import initialize_2D_radial
+
+    Nc, Ns = 16, 33
+    t = initialize_2D_radial(Nc, Ns)
+    r = 0.5 * np.linspace(0, 1, Ns)
+    th = 2 * np.pi * np.arange(Nc) / Nc
+    ref = np.stack([np.cos(th)[:, None] * r, np.sin(th)[:, None] * r], -1)
+    np.testing.assert_allclose(t, ref, atol=1e-13)
+    t2 = initialize_2D_radial(8, 9, in_out=True)
+    np.testing.assert_allclose(np.linalg.norm(t2, axis=-1).max(), 0.5)
+    np.testing.assert_allclose(t2[:, 4], 0, atol=1e-15)
+
+
+def test_spiral_closed_form():
+    from hipnufft.trajectories import initialize_2D_spiral
+
+    Nc, Ns, rev = 4, 65, 2
+    t = initialize_2D_spiral(Nc, Ns, nb_revolutions=rev)
+    a = 2 * np.pi * rev * np.linspace(0, 1, Ns)
+    shot = a * np.exp(1j * a) / a.max() * 0.5
+    for i in range(Nc):
+        z = shot * np.exp(1j * 2 * np.pi * i / Nc)
+        np.testing.assert_allclose(t[i, :, 0] + 1j * t[i, :, 1], z, atol=1e-13)
+
+
+def test_phyllotaxis_closed_form():
+    from hipnufft.trajectories import GOLDEN, initialize_3D_phyllotaxis_radial
+
+    Nc, Ns = 64, 17
+    t = initialize_3D_phyllotaxis_radial(Nc, Ns)
+    z = np.linspace(-1, 1, Nc)
+    phi = GOLDEN * np.arange(Nc)
+    v = np.stack([np.cos(phi) * np.sqrt(1 - z ** 2), np.sin(phi) * np.sqrt(1 - z ** 2), z], -1)
+    r = 0.5 * np.linspace(0, 1, Ns)
+    ref = -r[None, :, None] * v[:, None, :]
+    np.testing.assert_allclose(t, ref, atol=1e-12)
+
+
+def test_stack_closed_form():
+    from hipnufft.trajectories import initialize_2D_radial, stack
+
+    base = initialize_2D_radial(4, 5)
+    t = stack(base, 3)
+    assert t.shape == (12, 5, 3)
+    np.testing.assert_allclose(t[:4, :, :2], base)
+    np.testing.assert_allclose(np.unique(t[:, :, 2]), [-1 / 3, 0, 1 / 3], atol=1e-15)
+
+
+def test_smaps_phantom():
+    from hipnufft.trajectories import ellipsoid_phantom, gaussian_smaps
+
+    s = gaussian_smaps(5, (8, 9, 10), seed=3)
+    np.testing.assert_allclose(np.sum(np.abs(s.astype(np.complex128)) ** 2, 0), 1, rtol=1e-5)
+    p1 = ellipsoid_phantom((16, 16), seed=2)
+    p2 = ellipsoid_phantom((16, 16), seed=2)
+    np.testing.assert_array_equal(p1, p2)
+    assert p1.dtype == np.complex64
+
+
+def test_bench_refuses_multi_gpu():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2 and "one GPU" in r.stderr

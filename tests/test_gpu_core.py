@@ -59,10 +59,11 @@ def test_sort_bitexact(name, shape, mk):
     np.testing.assert_array_equal(ls.cpu().numpy(), nufft.bin_start(lk, nbins))
 
 
+@pytest.mark.parametrize("split", [0, 64], ids=["whole-tiles", "split-dense-tiles"])
 @pytest.mark.parametrize("name,shape,mk", CASES, ids=[c[0] for c in CASES])
-def test_type2_type1_vs_oracle(name, shape, mk):
+def test_type2_type1_vs_oracle(name, shape, mk, split):
     traj = mk()
-    p, om = plan_for(traj, shape)
+    p, om = plan_for(traj, shape, spread_max=split, max_subproblem=split)
     x = rand_cplx(shape, 10)
     c = rand_cplx(len(traj), 11)
     y = torch.empty(len(traj), dtype=torch.complex64, device=DEV)
@@ -99,7 +100,7 @@ def test_sense_op_adj_dc(n_trans):
     shape = (48, 40)
     B, C = 2, 4
     traj = truncnorm_traj(1200, 2, 8)
-    p, om = plan_for(traj, shape, n_trans=n_trans)
+    p, om = plan_for(traj, shape, n_trans=n_trans, spread_max=100)
     M = len(traj)
     x = rand_cplx((B, *shape), 20)
     s = gauss_smaps(C, shape, 21)
