@@ -195,6 +195,7 @@ void free_plan(hipnufft_plan_s* p) {
   (void)hipFree(p->part);
   (void)hipFree(p->grid);
   (void)hipFree(p->kwork);
+  (void)hipFree(p->ksorted);
   delete p;
 }
 
@@ -206,19 +207,28 @@ int need_pts(const hipnufft_plan_s* p) {
 
 int64_t img_size(const hipnufft_plan_s* p) { return p->N[0] * p->N[1] * p->N[2]; }
 
-int ensure_kwork(hipnufft_plan_s* p) {
+// (re)allocate an ntrans * M complex workspace of the plan
+int ensure_kbuf(hipnufft_plan_s* p, float2** buf, int64_t* cap, const char* what) {
   const int64_t need = (int64_t)p->ntrans * std::max<int64_t>(p->M, 1);
-  if (p->kwork && p->kwork_cap >= need) return 0;
-  if (p->kwork) {
-    HN_HIP(hipFree(p->kwork));
-    p->bytes -= sizeof(float2) * p->kwork_cap;
+  if (*buf && *cap >= need) return 0;
+  if (*buf) {
+    HN_HIP(hipFree(*buf));
+    p->bytes -= sizeof(float2) * *cap;
   }
-  p->kwork = nullptr;
-  if (hipMalloc((void**)&p->kwork, sizeof(float2) * need) != hipSuccess)
-    return set_error(HIPNUFFT_ERR_ALLOC, "cannot allocate the residual workspace");
-  p->kwork_cap = need;
+  *buf = nullptr;
+  if (hipMalloc((void**)buf, sizeof(float2) * need) != hipSuccess)
+    return set_error(HIPNUFFT_ERR_ALLOC, std::string("cannot allocate the ") + what);
+  *cap = need;
   p->bytes += sizeof(float2) * need;
   return 0;
+}
+
+int ensure_kwork(hipnufft_plan_s* p) {
+  return ensure_kbuf(p, &p->kwork, &p->kwork_cap, "residual workspace");
+}
+
+int ensure_ksorted(hipnufft_plan_s* p) {
+  return ensure_kbuf(p, &p->ksorted, &p->ksorted_cap, "sorted k-space workspace");
 }
 
 // chunk size for B*C transforms: whole batches when they fit
@@ -459,6 +469,7 @@ int hipnufft_sense_adj(hipnufft_plan* p, const void* ksp, const void* smaps, con
   const float2* s = (const float2*)smaps;
   float2* x = (float2*)img;
   const int TT = B * C;
+  if ((rc = ensure_ksorted(p))) return rc;
   if (p->sio) {
     if (s) return set_error(HIPNUFFT_ERR_ARG, "spread/interp-only plans take no smaps");
     if (scale != 1.0f) return set_error(HIPNUFFT_ERR_ARG, "spread/interp-only adjoint has no scale");
@@ -500,7 +511,7 @@ int hipnufft_sense_dc(hipnufft_plan* p, const void* img, const void* smaps, cons
     if ((rc = launch_interp(p, p->grid, grid_size(p), Tc, p->kwork, y + (int64_t)t0 * p->M, density,
                             scale, 1)))
       return rc;
-    if ((rc = launch_spread(p, p->kwork, nullptr, Tc, p->grid, grid_size(p)))) return rc;
+    if ((rc = launch_spread_sorted(p, p->kwork, Tc, p->grid, grid_size(p)))) return rc;
     if ((rc = grid_to_img(p, g, s, C, t0, Tc, scale, -p->sign2))) return rc;
   }
   return 0;

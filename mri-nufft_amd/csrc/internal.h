@@ -133,8 +133,10 @@ struct hipnufft_plan_s {
 
   // workspace
   float2* grid = nullptr;   // ntrans * prod(nf)
-  float2* kwork = nullptr;  // ntrans * M (data-consistency residual)
+  float2* kwork = nullptr;  // ntrans * M (data-consistency residual, sorted order)
   int64_t kwork_cap = 0;
+  float2* ksorted = nullptr;  // ntrans * M (spreading input, sorted order)
+  int64_t ksorted_cap = 0;
   size_t bytes = 0;
 };
 
@@ -164,15 +166,20 @@ int build_points(hipnufft_plan_s* p, int64_t M, const float* const* xd);
 // ---- spread_interp.hip ---------------------------------------------------
 // Interpolation over transforms [0, T): out[t][idx] (original order).
 // mode 0: out = scale * interp
-// mode 1: out = (scale * interp - obs[t][idx]) * (density ? density[idx] : 1)
+// mode 1: out[t][q] = (scale * interp - obs[t][idx]) * (density ? density[idx] : 1)
+//         written at the sorted position q (input of launch_spread_sorted)
 int launch_interp(const hipnufft_plan_s* p, const float2* grid,
                   int64_t grid_stride, int T, float2* out, const float2* obs,
                   const float* density, float scale, int mode);
 // Spreading of in[t][idx] * (density ? density[idx] : 1) for t < T into the
-// grid (every cell of every transform written once; no pre-zeroing).
+// grid (every cell of every transform written once; no pre-zeroing).  The
+// strengths are first gathered into the plan's sorted-order workspace.
 int launch_spread(const hipnufft_plan_s* p, const float2* in,
                   const float* density, int T, float2* grid,
                   int64_t grid_stride);
+// Same, for strengths already in sorted-point order (density applied).
+int launch_spread_sorted(const hipnufft_plan_s* p, const float2* in_sorted,
+                         int T, float2* grid, int64_t grid_stride);
 // Trajectory gradient by the kernel derivative (type-2 direction).
 int launch_grad_traj(const hipnufft_plan_s* p, const float2* grid,
                      int64_t grid_stride, int T, const float2* dy,

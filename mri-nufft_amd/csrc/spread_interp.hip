@@ -6,11 +6,12 @@
 // (operators/interfaces/finufft.py:64-76).
 //
 // Spreading is output-driven: one workgroup owns one tile of the fine grid
-// for a group of transforms, accumulates the footprints of every point that
-// touches the tile (the spread lists built at setpts, one entry per
-// point/tile pair) into LDS with ds_add_f32, and writes the finished tile to
-// HBM exactly once with coalesced stores -- no global atomics and no
-// zero-fill pass over the grid.
+// (or one slice of a dense tile's list) for a group of transforms, sums the
+// footprints of every point that touches the tile (the spread lists built at
+// setpts, one entry per point/tile pair) in registers, each thread owning a
+// few cells, and writes the finished tile to HBM exactly once with coalesced
+// stores -- no atomics, no zero-fill pass over the grid, and a fixed
+// summation order.
 //
 // Interpolation stages the tile plus a w-wide halo in LDS and gathers each
 // sorted point's w^D footprint from LDS.  Dense bins are split into work
@@ -151,21 +152,21 @@ __global__ __launch_bounds__(kThreads) void k_spread(SpreadArgs a) {
     if ((int)threadIdx.x < n) {
       float* s = smem + threadIdx.x * PW;
       const int p = a.list_pts[eb + threadIdx.x];
-      const int idx = a.g.idx[p];
 #pragma unroll
       for (int d = 0; d < D; ++d) {
         const float off = a.g.off[d][p];
         reinterpret_cast<int*>(s)[d] = a.g.cell[d][p] - org[d];  // raw start, mod nf
         for (int j = 0; j < W; ++j) s[D + d * W + j] = es_eval(off - (float)j, inv_hw, beta);
       }
-      const float dens = a.density ? a.density[idx] : 1.0f;
+      // strengths are in sorted-point order (density already applied), so
+      // the entries of one tile read nearly contiguous runs
       float* sv = s + D + D * W;
 #pragma unroll
       for (int g = 0; g < G; ++g) {
         float2 v = make_float2(0.f, 0.f);
-        if (g < Gv) v = a.in[(int64_t)(t0 + g) * a.in_stride + idx];
-        sv[2 * g] = v.x * dens;
-        sv[2 * g + 1] = v.y * dens;
+        if (g < Gv) v = a.in[(int64_t)(t0 + g) * a.in_stride + p];
+        sv[2 * g] = v.x;
+        sv[2 * g + 1] = v.y;
       }
     }
     __syncthreads();
@@ -377,14 +378,17 @@ __global__ __launch_bounds__(kThreads) void k_interp(InterpArgs a) {
           sy += w0 * py;
         }
       }
-      const int64_t o = (int64_t)(t0 + gg) * a.out_stride + idx;
+      const int64_t row = (int64_t)(t0 + gg) * a.out_stride;
       float2 r = make_float2(a.scale * sx, a.scale * sy);
       if (a.mode == 1) {
-        const float2 y = a.obs[o];
+        // residual for the spreading that follows: written in sorted order
+        const float2 y = a.obs[row + idx];
         r.x = (r.x - y.x) * dens;
         r.y = (r.y - y.y) * dens;
+        a.out[row + p] = r;
+      } else {
+        a.out[row + idx] = r;
       }
-      a.out[o] = r;
     }
   }
 }
@@ -551,9 +555,47 @@ static int tile_cells(const hipnufft_plan_s* p, int halo) {
   return c;
 }
 
+namespace {
+
+// ksorted[t][q] = in[t][idx[q]] * density[idx[q]]: one pass that reads the
+// caller's k-space once (runs of consecutive samples stay contiguous after
+// binning) and writes the spreading input in sorted-point order.
+__global__ __launch_bounds__(kThreads) void k_gather_sorted(const float2* in, const float* density,
+                                                            const int32_t* idx, int64_t M,
+                                                            float2* out) {
+  const int64_t q = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (q >= M) return;
+  const int64_t t = blockIdx.y;
+  const int32_t o = idx[q];
+  float2 v = in[t * M + o];
+  if (density) {
+    const float d = density[o];
+    v.x *= d;
+    v.y *= d;
+  }
+  out[t * M + q] = v;
+}
+
+}  // namespace
+
 int launch_spread(const hipnufft_plan_s* p, const float2* in, const float* density, int T,
                   float2* grid, int64_t grid_stride) {
   if (T <= 0) return 0;
+  if (!p->ksorted || p->ksorted_cap < (int64_t)T * p->M)
+    return set_error(HIPNUFFT_ERR_ARG, "sorted k-space workspace missing");
+  if (p->M > 0) {
+    const dim3 g((unsigned)((p->M + kThreads - 1) / kThreads), (unsigned)T);
+    hipLaunchKernelGGL(k_gather_sorted, g, dim3(kThreads), 0, p->stream, in, density, p->pt_idx,
+                       p->M, p->ksorted);
+    HN_LAUNCH_CHECK();
+  }
+  return launch_spread_sorted(p, p->ksorted, T, grid, grid_stride);
+}
+
+int launch_spread_sorted(const hipnufft_plan_s* p, const float2* in, int T, float2* grid,
+                         int64_t grid_stride) {
+  if (T <= 0) return 0;
+  const float* density = nullptr;
   SpreadArgs a{};
   a.g = pts_geom(p);
   a.list_pts = p->list_pts;

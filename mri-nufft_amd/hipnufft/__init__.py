@@ -42,10 +42,16 @@ def register_into_mrinufft():
         import mrinufft.operators.base as mb
     except ImportError:
         return False
-    if "hipnufft" in mb.FourierOperatorBase.interfaces:
+    registry = mb.FourierOperatorBase.__dict__["interfaces"]
+    if "hipnufft" in registry:
         return True
-    # the class body registers itself through FourierOperatorBase.__init_subclass__
-    type("MRIHipNUFFT", (MRIHipNUFFT, mb.FourierOperatorBase),
-         {"backend": "hipnufft", "available": MRIHipNUFFT.available,
-          "__module__": "mrinufft.operators.interfaces.hipnufft"})
+    own = FourierOperatorBase.interfaces.get("hipnufft")
+    cls = type("MRIHipNUFFT", (MRIHipNUFFT, mb.FourierOperatorBase),
+               {"backend": "hipnufft", "available": MRIHipNUFFT.available,
+                "__module__": "mrinufft.operators.interfaces.hipnufft"})
+    if own is not None:  # keep hipnufft's own registry pointing at the plain class
+        FourierOperatorBase.interfaces["hipnufft"] = own
+    # hipnufft's base comes first in the MRO, so ``cls.interfaces`` is
+    # hipnufft's own dict: register in mri-nufft's registry explicitly
+    registry["hipnufft"] = (MRIHipNUFFT.available, cls)
     return True

@@ -161,3 +161,46 @@ def test_bench_refuses_multi_gpu():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+def test_register_into_mrinufft(tmp_path, monkeypatch):
+    """A minimal stand-in package with mri-nufft's registry contract
+    (operators/base.py:182-218: abstract op/adj_op/_op/_adj_op, a class-level
+    ``interfaces`` dict filled by ``__init_subclass__``)."""
+    pkg = tmp_path / "mrinufft" / "operators"
+    pkg.mkdir(parents=True)
+    (tmp_path / "mrinufft" / "__init__.py").write_text("")
+    (pkg / "__init__.py").write_text("")
+    (pkg / "base.py").write_text(
+        "from abc import ABC, abstractmethod\n"
+        "class FourierOperatorBase(ABC):\n"
+        "    interfaces = {}\n"
+        "    def __init_subclass__(cls):\n"
+        "        super().__init_subclass__()\n"
+        "        name = getattr(cls, 'backend', None)\n"
+        "        if name:\n"
+        "            cls.interfaces[name] = (getattr(cls, 'available', True), cls)\n"
+        "    @abstractmethod\n"
+        "    def op(self, x): ...\n"
+        "    @abstractmethod\n"
+        "    def adj_op(self, y): ...\n"
+        "    @abstractmethod\n"
+        "    def _op(self, x, y): ...\n"
+        "    @abstractmethod\n"
+        "    def _adj_op(self, y, x): ...\n")
+    monkeypatch.syspath_prepend(str(tmp_path))
+    for m in [m for m in sys.modules if m.startswith("mrinufft")]:
+        monkeypatch.delitem(sys.modules, m)
+    import hipnufft
+    from hipnufft.base import FourierOperatorBase as OwnBase
+
+    own_before = OwnBase.interfaces["hipnufft"]
+    assert hipnufft.register_into_mrinufft() is True
+    import mrinufft.operators.base as mb
+
+    avail, cls = mb.FourierOperatorBase.interfaces["hipnufft"]
+    assert issubclass(cls, mb.FourierOperatorBase) and issubclass(cls, hipnufft.MRIHipNUFFT)
+    assert not getattr(cls, "__abstractmethods__", None)
+    assert avail == hipnufft.MRIHipNUFFT.available
+    assert OwnBase.interfaces["hipnufft"] == own_before
+    assert hipnufft.register_into_mrinufft() is True  # idempotent
