@@ -164,6 +164,9 @@ def main():
     ap.add_argument("--cg-iter", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--plan-opt", action="append", default=[], metavar="KEY=INT",
+                    help="plan tuning option passed to the operator (spread_group, "
+                         "interp_group, max_subproblem, spread_max)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); this build measures "
@@ -185,7 +188,8 @@ def main():
     smaps_np = gaussian_smaps(C, shape, seed=1)
     dev = torch.device("cuda", 0)
     t = time.perf_counter()
-    op = hipnufft.get_operator("hipnufft")(traj, shape, n_coils=C, smaps=smaps_np)
+    plan_opts = {k: int(v) for k, v in (o.split("=", 1) for o in args.plan_opt)}
+    op = hipnufft.get_operator("hipnufft")(traj, shape, n_coils=C, smaps=smaps_np, **plan_opts)
     torch.cuda.synchronize()
     t_plan = time.perf_counter() - t
     log(f"operator ready in {t_plan:.2f}s; n_trans={op.n_trans}; info={op.raw_op.plan.info()}")
@@ -272,7 +276,7 @@ def main():
         "data": "synthetic (seeded phyllotaxis trajectory, ellipsoid phantom, Gaussian coil maps)",
         "config": {"workload": "3D phyllotaxis radial 4096x512 (2,097,152 pts), 192^3, "
                                "16 coils SENSE, eps 1e-6, sigma 2",
-                   "n_trans": T, "plan_setup_s": round(t_plan, 3)},
+                   "n_trans": T, "plan_setup_s": round(t_plan, 3), "plan_opts": plan_opts},
         "cg_latency_ms": cg,
         "accuracy": acc,
         "stages": stage_summary,

@@ -408,6 +408,10 @@ int hipnufft_plan_create(int type, int ndim, const int64_t* nmodes, int iflag, i
   }
   if (!rc && !p->sio) {
     for (int d = off; d < 3 && !rc; ++d) rc = make_axis(p->nf[d], &p->fft[d], &p->bytes);
+    // the fastest axis' adjoint kernel also holds its coil sum in LDS
+    while (!rc && p->fft[2].L > 1 && rows_adj_lds(p->fft[2], p->N[2]) > 65536) p->fft[2].L >>= 1;
+    if (!rc && rows_adj_lds(p->fft[2], p->N[2]) > 65536)
+      rc = set_error(HIPNUFFT_ERR_ARG, "fine grid too long along the last axis (max 2048)");
     if (!rc) {
       const size_t g = sizeof(float2) * (size_t)grid_size(p) * (size_t)ntrans;
       if (hipMalloc((void**)&p->grid, g) != hipSuccess)

@@ -227,22 +227,36 @@ __global__ __launch_bounds__(kThreads) void k_rows_fwd(RowsArgs a) {
   const float2* sm = a.smaps ? a.smaps + (int64_t)(tt % a.C) * N : nullptr;
   const int n = a.nf2;
 
-  for (int idx = threadIdx.x; idx < a.M2 * L; idx += kThreads) {
+  // loads first (overlapping), then the arithmetic and LDS stores
+  float2 v[kEPT], s[kEPT];
+#pragma unroll
+  for (int i = 0; i < kEPT; ++i) {
+    const int idx = threadIdx.x + i * kThreads;
     const int l = idx / a.M2, x = idx - l * a.M2;
     const int r = r0 + l;
-    float2 v = make_float2(0.f, 0.f);
-    if (r < rows) {
-      const int i0 = r / a.M1, i1 = r - i0 * a.M1;
+    v[i] = make_float2(0.f, 0.f);
+    s[i] = make_float2(1.f, 0.f);
+    if (idx < a.M2 * L && r < rows) {
       const int64_t off = (int64_t)r * a.M2 + x;
-      v = src[off];
-      if (sm) {
-        float2 s = sm[off];
-        if (a.conj_smaps) s.y = -s.y;
-        v = cmul(v, s);
-      }
-      v = cscale(v, a.dec0[i0] * a.dec1[i1] * a.dec2[x] * a.scale);
+      v[i] = src[off];
+      if (sm) s[i] = sm[off];
     }
-    buf[kept_pos(x, a.M2, n) * LP + l] = v;
+  }
+#pragma unroll
+  for (int i = 0; i < kEPT; ++i) {
+    const int idx = threadIdx.x + i * kThreads;
+    if (idx < a.M2 * L) {
+      const int l = idx / a.M2, x = idx - l * a.M2;
+      const int r = r0 + l;
+      float2 w = make_float2(0.f, 0.f);
+      if (r < rows) {
+        const int i0 = r / a.M1, i1 = r - i0 * a.M1;
+        float2 sv = s[i];
+        if (a.conj_smaps) sv.y = -sv.y;
+        w = cscale(cmul(v[i], sv), a.dec0[i0] * a.dec1[i1] * a.dec2[x] * a.scale);
+      }
+      buf[kept_pos(x, a.M2, n) * LP + l] = w;
+    }
   }
   const int gap0 = a.M2 - a.M2 / 2, gap = n - a.M2;
   for (int idx = threadIdx.x; idx < gap * L; idx += kThreads) {
@@ -266,8 +280,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_fwd(RowsArgs a) {
 // grid rows -> FFT -> crop -> deconvolve * scale (* conj smap, summed over the
 // chunk's coils of one batch) -> image rows
 template <int S>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) void k_rows_adj(
-    RowsArgs a) {
+__global__ __launch_bounds__(kThreads) void k_rows_adj(RowsArgs a) {
   extern __shared__ float2 buf[];
   const int L = 1 << a.lgL, LP = L + 1;
   const float2* twl = stage_twiddles(buf, a.ax, LP);
@@ -287,61 +300,70 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
     te = out_img + 1;
   }
   const bool add_existing = a.smaps && (tb % a.C != 0);
-  const int nout = a.M2 * L;
-  float2 acc[kEPT];
-#pragma unroll
-  for (int i = 0; i < kEPT; ++i) acc[i] = make_float2(0.f, 0.f);
+  // line-major thread mapping: thread (l, x0) handles positions x0 + j * TPL
+  // of line l, so it keeps one row base across the coil loop
+  const int TPL = kThreads >> a.lgL;  // threads per line (n <= TPL * kEPT)
+  const int l = threadIdx.x / TPL, x0 = threadIdx.x - l * TPL;
+  const int r = r0 + l;
+  const bool rok = r < rows;
+  int64_t grow = 0;
+  if (rok) {
+    const int i0 = r / a.M1, i1 = r - i0 * a.M1;
+    grow = ((int64_t)kept_pos(i0, a.M0, a.nf0) * a.nf1 + kept_pos(i1, a.M1, a.nf1)) * n;
+  }
+  // coil sum of the cropped lines in LDS after the line data and twiddles;
+  // each entry is only touched by the thread that owns it
+  float2* accl = buf + (size_t)n * (LP + 1) + (size_t)l * a.M2;
 
   for (int tt = tb; tt < te; ++tt) {
-    const float2* g = a.grid + (int64_t)(tt - a.t0) * a.gstride;
-    for (int idx = threadIdx.x; idx < n * L; idx += kThreads) {
-      const int l = idx / n, x = idx - l * n;
-      const int r = r0 + l;
-      float2 v = make_float2(0.f, 0.f);
-      if (r < rows) {
-        const int i0 = r / a.M1, i1 = r - i0 * a.M1;
-        const int64_t grow = ((int64_t)kept_pos(i0, a.M0, a.nf0) * a.nf1 + kept_pos(i1, a.M1, a.nf1)) * n;
-        v = g[grow + x];
+    const float2* g = a.grid + (int64_t)(tt - a.t0) * a.gstride + grow;
+    {
+      float2 v[kEPT];
+#pragma unroll
+      for (int j = 0; j < kEPT; ++j) {
+        const int x = x0 + j * TPL;
+        v[j] = (rok && x < n) ? g[x] : make_float2(0.f, 0.f);
       }
-      buf[x * LP + l] = v;
+#pragma unroll
+      for (int j = 0; j < kEPT; ++j) {
+        const int x = x0 + j * TPL;
+        if (x < n) buf[x * LP + l] = v[j];
+      }
     }
     __syncthreads();
-    fft_lines<S>(buf, twl, a.ax, a.lgL, LP);
-    const float2* sm = a.smaps ? a.smaps + (int64_t)(tt % a.C) * N : nullptr;
+    // the FFT's addressing is the same for every coil; hiding the line
+    // geometry from the optimiser keeps it from hoisting all of it out of the
+    // coil loop into registers
+    int lg = a.lgL, lp = LP;
+    asm volatile("" : "+s"(lg), "+s"(lp));
+    fft_lines<S>(buf, twl, a.ax, lg, lp);
+    const float2* sm = a.smaps ? a.smaps + (int64_t)(tt % a.C) * N + (int64_t)r * a.M2 : nullptr;
 #pragma unroll
-    for (int i = 0; i < kEPT; ++i) {
-      const int idx = threadIdx.x + i * kThreads;
-      if (idx < nout) {
-        const int l = idx / a.M2, x = idx - l * a.M2;
-        const int r = r0 + l;
-        if (r < rows) {
-          float2 v = buf[kept_pos(x, a.M2, n) * LP + l];
+    for (int j = 0; j < kEPT; ++j) {
+      const int x = x0 + j * TPL;
+      if (x < a.M2) {
+        float2 v = make_float2(0.f, 0.f);
+        if (rok) {
+          v = buf[kept_pos(x, a.M2, n) * LP + l];
           if (sm) {
-            float2 s = sm[(int64_t)r * a.M2 + x];
-            s.y = -s.y;
-            v = cmul(v, s);
+            float2 sv = sm[x];
+            sv.y = -sv.y;
+            v = cmul(v, sv);
           }
-          acc[i] = cadd(acc[i], v);
         }
+        accl[x] = (tt == tb) ? v : cadd(accl[x], v);
       }
     }
     __syncthreads();
   }
-  float2* dst = a.img_out + (int64_t)out_img * N;
-#pragma unroll
-  for (int i = 0; i < kEPT; ++i) {
-    const int idx = threadIdx.x + i * kThreads;
-    if (idx < nout) {
-      const int l = idx / a.M2, x = idx - l * a.M2;
-      const int r = r0 + l;
-      if (r < rows) {
-        const int i0 = r / a.M1, i1 = r - i0 * a.M1;
-        float2 v = cscale(acc[i], a.dec0[i0] * a.dec1[i1] * a.dec2[x] * a.scale);
-        const int64_t o = (int64_t)r * a.M2 + x;
-        if (add_existing) v = cadd(v, dst[o]);
-        dst[o] = v;
-      }
-    }
+  if (!rok) return;
+  const int i0 = r / a.M1, i1 = r - i0 * a.M1;
+  const float rs = a.dec0[i0] * a.dec1[i1] * a.scale;
+  float2* dst = a.img_out + (int64_t)out_img * N + (int64_t)r * a.M2;
+  for (int x = x0; x < a.M2; x += TPL) {
+    float2 v = cscale(accl[x], rs * a.dec2[x]);
+    if (add_existing) v = cadd(v, dst[x]);
+    dst[x] = v;
   }
 }
 
@@ -373,10 +395,27 @@ __global__ __launch_bounds__(kThreads) void k_fft_strided(StridedArgs a) {
   float2* base = a.data + (int64_t)t * a.tstride + (int64_t)opos * a.ostride + x0;
   const int n = a.n;
   const int nin = a.in_M ? a.in_M : n;
-  for (int idx = threadIdx.x; idx < nin * L; idx += kThreads) {
+  // all of a thread's loads are issued before the first LDS store, so they
+  // overlap instead of paying one memory latency each (n * L <= kEPT * kThreads)
+  float2 v[kEPT];
+#pragma unroll
+  for (int i = 0; i < kEPT; ++i) {
+    const int idx = threadIdx.x + i * kThreads;
     const int l = idx & (L - 1), e = idx >> a.lgL;
-    const int pos = a.in_M ? kept_pos(e, a.in_M, n) : e;
-    buf[pos * LP + l] = (x0 + l < a.nx) ? base[(int64_t)pos * a.lstride + l] : make_float2(0.f, 0.f);
+    v[i] = make_float2(0.f, 0.f);
+    if (idx < nin * L && x0 + l < a.nx) {
+      const int pos = a.in_M ? kept_pos(e, a.in_M, n) : e;
+      v[i] = base[(int64_t)pos * a.lstride + l];
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < kEPT; ++i) {
+    const int idx = threadIdx.x + i * kThreads;
+    if (idx < nin * L) {
+      const int l = idx & (L - 1), e = idx >> a.lgL;
+      const int pos = a.in_M ? kept_pos(e, a.in_M, n) : e;
+      buf[pos * LP + l] = v[i];
+    }
   }
   if (a.in_M) {
     const int gap0 = a.in_M - a.in_M / 2, gap = n - a.in_M;
@@ -402,6 +441,15 @@ int lg2(int x) {
 }
 
 size_t fft_lds(const FFTAxis& ax) { return sizeof(float2) * (size_t)ax.n * (ax.L + 2); }
+
+}  // namespace
+
+// k_rows_adj also keeps the coil sum of its L cropped lines in LDS
+size_t rows_adj_lds(const FFTAxis& ax, int64_t m2) {
+  return fft_lds(ax) + sizeof(float2) * (size_t)m2 * ax.L;
+}
+
+namespace {
 
 int strided(const hipnufft_plan_s* p, int axis, int T, int in_M, int out_M, int outer_M,
             int sign, int stage) {
@@ -510,7 +558,7 @@ int grid_to_img(const hipnufft_plan_s* p, float2* img, const float2* smaps, int 
   const int L = p->fft[2].L;
   const int nimg = smaps ? ((t0 + T - 1) / C - t0 / C + 1) : T;
   const dim3 grd((unsigned)((rows + L - 1) / L), (unsigned)nimg);
-  const size_t lds = fft_lds(p->fft[2]);
+  const size_t lds = rows_adj_lds(p->fft[2], a.M2);
   StageScope timer(p, kStageRowsAdj);
   if (sign < 0)
     hipLaunchKernelGGL(k_rows_adj<-1>, grd, dim3(kThreads), lds, p->stream, a);

@@ -157,6 +157,8 @@ class StageScope {
 
 int64_t grid_size(const hipnufft_plan_s* p);
 PtsGeom pts_geom(const hipnufft_plan_s* p);
+// dynamic LDS bytes of the last-axis adjoint kernel (lines, twiddles, coil sum)
+size_t rows_adj_lds(const FFTAxis& ax, int64_t m2);
 
 // ---- sort.hip -------------------------------------------------------------
 // Bins the points and builds the sorted arrays, interp work items and

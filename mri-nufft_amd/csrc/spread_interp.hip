@@ -92,22 +92,27 @@ __device__ __forceinline__ void tile_coords(int key, const int* nt, int* tc) {
 // ---------------------------------------------------------------------------
 // spreading: owner-computes gather, no atomics
 // ---------------------------------------------------------------------------
-// Each thread owns up to kCellsPerThread cells of the tile and keeps their
-// sums for the G transforms in registers.  The tile's list entries are staged
-// through LDS kSpreadChunk at a time (footprint start relative to the tile,
-// the w weights per axis, the G strengths); every thread then walks the
-// staged points and adds the ones whose footprint covers its cells.  Points
-// read from LDS are wave-uniform (broadcast), and each cell's sum is formed
-// in list order, so the result is deterministic bit for bit.  Thousands of
-// radial spokes crossing the k-space centre cost no contention.
-constexpr int kCellsPerThread = 4;  // tiles hold at most 1024 cells
+// Cell ownership: the tile is a set of columns along the last (contiguous)
+// axis; thread (c, s) owns kZ consecutive cells z0 = s * kZ ... of column c
+// and keeps their sums for the G transforms in registers.  With 64 columns
+// (8 x 8 x 16 tiles) a wave holds one segment s of every column, so a point
+// whose footprint misses the segment is skipped by the whole wave at once,
+// and the column weight is formed once per point for the kZ cells.
+//
+// The tile's list entries are staged through LDS kSpreadChunk at a time
+// (footprint start relative to the tile, the w weights per axis, the G
+// strengths); every thread then walks the staged points in list order, so
+// each cell's sum is formed in the same order on every run: deterministic
+// bit for bit, and thousands of radial spokes crossing the k-space centre
+// cost no contention.
+constexpr int kZ = 4;  // cells per thread along the last axis
 constexpr int kSpreadChunk = kThreads;
 
 template <int D, int G>
 __global__ __launch_bounds__(kThreads) void k_spread(SpreadArgs a) {
   extern __shared__ float smem[];
   const int W = a.g.w;
-  const int PW = D * W + D + 2 * G;  // floats per staged point
+  const int PW = D + D * W + 2 * G;  // floats per staged point
   const int item = blockIdx.x;
   const int tile = a.it_tile[item];
   const int slot = a.it_slot[item];
@@ -126,41 +131,48 @@ __global__ __launch_bounds__(kThreads) void k_spread(SpreadArgs a) {
     ext[d] = min(td[d], nf[d] - org[d]);
     TC *= td[d];
   }
-  int cl[kCellsPerThread][3];
-  bool own[kCellsPerThread];
+  constexpr int DL = D - 1;  // last axis
+  const int zl = td[DL];
+  const int ncol = TC / zl;
+  const int col = threadIdx.x % ncol;
+  const int z0 = (threadIdx.x / ncol) * kZ;
+  int cc[3] = {0, 0, 0};
+  bool colown = true;
+  {
+    int r = col;
 #pragma unroll
-  for (int k = 0; k < kCellsPerThread; ++k) {
-    int r = threadIdx.x + k * kThreads;
-    own[k] = r < TC;
-#pragma unroll
-    for (int d = D - 1; d >= 0; --d) {
-      cl[k][d] = r % td[d];
+    for (int d = DL - 1; d >= 0; --d) {
+      cc[d] = r % td[d];
       r /= td[d];
-      own[k] = own[k] && cl[k][d] < ext[d];
+      colown = colown && cc[d] < ext[d];
     }
   }
-  float2 acc[kCellsPerThread][G];
+  bool own[kZ];
 #pragma unroll
-  for (int k = 0; k < kCellsPerThread; ++k)
+  for (int j = 0; j < kZ; ++j) own[j] = colown && z0 + j < ext[DL];
+
+  float2 acc[kZ][G];
 #pragma unroll
-    for (int g = 0; g < G; ++g) acc[k][g] = make_float2(0.f, 0.f);
+  for (int j = 0; j < kZ; ++j)
+#pragma unroll
+    for (int g = 0; g < G; ++g) acc[j][g] = make_float2(0.f, 0.f);
 
   const int e0 = a.it_begin[item];
   const int e1 = min(e0 + a.smax, a.list_start[tile + 1]);
   for (int eb = e0; eb < e1; eb += kSpreadChunk) {
     const int n = min(kSpreadChunk, e1 - eb);
     if ((int)threadIdx.x < n) {
-      float* s = smem + threadIdx.x * PW;
+      float* sp = smem + threadIdx.x * PW;
       const int p = a.list_pts[eb + threadIdx.x];
 #pragma unroll
       for (int d = 0; d < D; ++d) {
         const float off = a.g.off[d][p];
-        reinterpret_cast<int*>(s)[d] = a.g.cell[d][p] - org[d];  // raw start, mod nf
-        for (int j = 0; j < W; ++j) s[D + d * W + j] = es_eval(off - (float)j, inv_hw, beta);
+        reinterpret_cast<int*>(sp)[d] = a.g.cell[d][p] - org[d];  // raw start, mod nf
+        for (int j = 0; j < W; ++j) sp[D + d * W + j] = es_eval(off - (float)j, inv_hw, beta);
       }
       // strengths are in sorted-point order (density already applied), so
       // the entries of one tile read nearly contiguous runs
-      float* sv = s + D + D * W;
+      float* sv = sp + D + D * W;
 #pragma unroll
       for (int g = 0; g < G; ++g) {
         float2 v = make_float2(0.f, 0.f);
@@ -171,51 +183,71 @@ __global__ __launch_bounds__(kThreads) void k_spread(SpreadArgs a) {
     }
     __syncthreads();
     for (int q = 0; q < n; ++q) {
-      const float* s = smem + q * PW;
-      int ls[3];
+      const float* sp = smem + q * PW;
+      const int* si = reinterpret_cast<const int*>(sp);
+      // last axis: which of the kZ cells the footprint covers
+      const int lsz = si[DL];
+      int dz[kZ];
+      bool hz[kZ];
+      bool any = false;
 #pragma unroll
-      for (int d = 0; d < D; ++d) ls[d] = reinterpret_cast<const int*>(s)[d];
-      const float* sv = s + D + D * W;
+      for (int j = 0; j < kZ; ++j) {
+        int da = z0 + j - lsz;  // owned cells satisfy z0 + j < nf: one wrap each way
+        if (da < 0) da += nf[DL];
+        if (da >= nf[DL]) da -= nf[DL];
+        hz[j] = own[j] && da < W;
+        dz[j] = hz[j] ? da : 0;
+        any = any || hz[j];
+      }
+      if (!any) continue;  // uniform across the wave for interior 3D tiles
+      float wc = 1.0f;
+      bool hc = true;
 #pragma unroll
-      for (int k = 0; k < kCellsPerThread; ++k) {
-        if (!own[k]) continue;
-        float wgt = 1.0f;
-        bool hit = true;
+      for (int d = 0; d < DL; ++d) {
+        int da = cc[d] - si[d];
+        if (da < 0) da += nf[d];
+        if (da >= nf[d]) da -= nf[d];
+        hc = hc && da < W;
+        wc *= sp[D + d * W + (hc ? da : 0)];  // index stays inside the record
+      }
+      if (!hc) wc = 0.0f;
+      const float* wl = sp + D + DL * W;
+      const float* sv = sp + D + D * W;
+      float2 v[G];
 #pragma unroll
-        for (int d = 0; d < D; ++d) {
-          int da = cl[k][d] - ls[d];
-          if (da < 0) da += nf[d];
-          if (da >= nf[d]) da -= nf[d];
-          hit = hit && da < W;
-          wgt *= s[D + d * W + (hit ? da : 0)];  // index stays inside the record
-        }
-        if (!hit) continue;
+      for (int g = 0; g < G; ++g) v[g] = make_float2(sv[2 * g], sv[2 * g + 1]);
+#pragma unroll
+      for (int j = 0; j < kZ; ++j) {
+        if (!hz[j]) continue;
+        const float f = wc * wl[dz[j]];
 #pragma unroll
         for (int g = 0; g < G; ++g) {
-          acc[k][g].x += wgt * sv[2 * g];
-          acc[k][g].y += wgt * sv[2 * g + 1];
+          acc[j][g].x += f * v[g].x;
+          acc[j][g].y += f * v[g].y;
         }
       }
     }
     __syncthreads();
   }
 
+  // tile-local linear index of cell (cc, z0 + j) is col * zl + z0 + j
 #pragma unroll
-  for (int k = 0; k < kCellsPerThread; ++k) {
-    if (!own[k]) continue;
+  for (int j = 0; j < kZ; ++j) {
+    if (!own[j]) continue;
     if (slot >= 0) {  // partial tile of a dense tile: summed by k_spread_reduce
-      const int r = threadIdx.x + k * kThreads;
+      const int r = col * zl + z0 + j;
 #pragma unroll
       for (int g = 0; g < G; ++g)
-        if (g < Gv) a.part[((int64_t)slot * a.ntrans + t0 + g) * TC + r] = acc[k][g];
+        if (g < Gv) a.part[((int64_t)slot * a.ntrans + t0 + g) * TC + r] = acc[j][g];
       continue;
     }
     int64_t gi = 0;
 #pragma unroll
-    for (int d = 0; d < D; ++d) gi = gi * nf[d] + org[d] + cl[k][d];
+    for (int d = 0; d < DL; ++d) gi = gi * nf[d] + org[d] + cc[d];
+    gi = gi * nf[DL] + org[DL] + z0 + j;
 #pragma unroll
     for (int g = 0; g < G; ++g)
-      if (g < Gv) a.grid[(int64_t)(t0 + g) * a.grid_stride + gi] = acc[k][g];
+      if (g < Gv) a.grid[(int64_t)(t0 + g) * a.grid_stride + gi] = acc[j][g];
   }
 }
 
@@ -274,23 +306,39 @@ __device__ __forceinline__ void stage_region(const InterpArgs& a, float2* lds, i
     rorg[d] = tc[d] * a.g.tile[d] - W / 2;
     RC *= R[d];
   }
-  for (int c = threadIdx.x; c < G * RC; c += kThreads) {
-    const int gg = c / RC;
-    int r = c - gg * RC;
-    int l[3];
+  // kStageBatch loads per thread are in flight before their LDS stores
+  constexpr int kStageBatch = 8;
+  const int total = G * RC;
+  for (int c0 = threadIdx.x; c0 < total; c0 += kStageBatch * kThreads) {
+    float2 v[kStageBatch];
 #pragma unroll
-    for (int d = D - 1; d >= 0; --d) {
-      l[d] = r % R[d];
-      r /= R[d];
-    }
-    int64_t gi = 0;
+    for (int u = 0; u < kStageBatch; ++u) {
+      const int c = c0 + u * kThreads;
+      if (c < total) {
+        const int gg = c / RC;
+        int r = c - gg * RC;
+        int l[3];
 #pragma unroll
-    for (int d = 0; d < D; ++d) {
-      int x = (rorg[d] + l[d]) % a.g.nf[d];
-      if (x < 0) x += a.g.nf[d];
-      gi = gi * a.g.nf[d] + x;
+        for (int d = D - 1; d >= 0; --d) {
+          l[d] = r % R[d];
+          r /= R[d];
+        }
+        int64_t gi = 0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          // the region can exceed the grid several times over (tiny grids)
+          int x = (rorg[d] + l[d]) % a.g.nf[d];
+          if (x < 0) x += a.g.nf[d];
+          gi = gi * a.g.nf[d] + x;
+        }
+        v[u] = a.grid[(int64_t)(t0 + gg) * a.grid_stride + gi];
+      }
     }
-    lds[c] = a.grid[(int64_t)(t0 + gg) * a.grid_stride + gi];
+#pragma unroll
+    for (int u = 0; u < kStageBatch; ++u) {
+      const int c = c0 + u * kThreads;
+      if (c < total) lds[c] = v[u];
+    }
   }
 }
 
@@ -616,10 +664,14 @@ int launch_spread_sorted(const hipnufft_plan_s* p, const float2* in, int T, floa
   a.grid_stride = grid_stride;
   a.T = T;
   if (T > p->ntrans) return set_error(HIPNUFFT_ERR_ARG, "spread chunk larger than ntrans");
-  if (tile_cells(p, 0) > kCellsPerThread * kThreads)
-    return set_error(HIPNUFFT_ERR_ARG, "spread tile larger than 1024 cells");
+  {  // every (column, kZ-cell segment) of a tile needs its own thread
+    const int zl = p->tile[2];
+    const int ncol = tile_cells(p, 0) / zl;
+    if (ncol * ((zl + kZ - 1) / kZ) > kThreads)
+      return set_error(HIPNUFFT_ERR_ARG, "spread tile has more than 256 column segments of 4 cells");
+  }
   int G = p->spread_group;
-  G = G >= 8 ? 8 : G >= 4 ? 4 : G >= 2 ? 2 : 1;
+  G = G >= 16 ? 16 : G >= 8 ? 8 : G >= 4 ? 4 : G >= 2 ? 2 : 1;
   while (G > 1 && G / 2 >= T) G /= 2;
   auto lds_of = [&](int g) {
     return sizeof(float) * (size_t)kSpreadChunk * (p->ndim * p->w + p->ndim + 2 * g);
@@ -631,6 +683,7 @@ int launch_spread_sorted(const hipnufft_plan_s* p, const float2* in, int T, floa
   StageScope timer(p, kStageSpread);
 #define HN_SPREAD(DD)                                                                        \
   switch (G) {                                                                               \
+    case 16: hipLaunchKernelGGL((k_spread<DD, 16>), grd, dim3(kThreads), lds, p->stream, a); break; \
     case 8: hipLaunchKernelGGL((k_spread<DD, 8>), grd, dim3(kThreads), lds, p->stream, a); break; \
     case 4: hipLaunchKernelGGL((k_spread<DD, 4>), grd, dim3(kThreads), lds, p->stream, a); break; \
     case 2: hipLaunchKernelGGL((k_spread<DD, 2>), grd, dim3(kThreads), lds, p->stream, a); break; \
