@@ -83,6 +83,21 @@ int hipnufft_plan_create(int type, int ndim, const int64_t* nmodes, int iflag,
 int hipnufft_setpts(hipnufft_plan* plan, int64_t M, const float* x0_d,
                     const float* x1_d, const float* x2_d);
 
+/* Set K point sets (trajectories) at once: set k has Ms[k] samples (host
+ * array), coordinates of all sets back to back in x_d[sum Ms].  All sets are
+ * binned and sorted together (set-major keys), so one launch per stage covers
+ * every set whatever its length.  With K > 1 the SENSE entry points take one
+ * batch per set (B == K) and ntrans >= C; batch b uses set b, k-space arrays
+ * are laid out [k][c][Ms[k]] back to back, smaps_d (if any) holds one map
+ * per set and coil [k][c][prod(nmodes)], and hipnufft_execute /
+ * execute_adjoint run one transform per set.  K == 1 is hipnufft_setpts.
+ * No counterpart in the reference, which plans one trajectory per operator
+ * (cufinufft.py:202-275); it replaces a Python loop over per-trajectory
+ * operators. */
+int hipnufft_setpts_sets(hipnufft_plan* plan, int nsets, const int64_t* Ms,
+                         const float* x0_d, const float* x1_d,
+                         const float* x2_d);
+
 /* Type 2 over ntrans transforms: in_d[ntrans][prod(nmodes)] ->
  * out_d[ntrans][M].  Replaces Plan.execute for type 2 (cufinufft.py:131-133)
  * and finufft Plan.execute (finufft.py:71-76). */
@@ -147,11 +162,22 @@ int hipnufft_sense_grad_traj(hipnufft_plan* plan, const void* img_d,
 int hipnufft_dot_c64(const void* a_d, const void* b_d, int64_t n, int conj,
                      double* out_d, double* ws_d, void* stream);
 
+/* rows independent dots of the same kind over a_d[rows][n], b_d[rows][n]:
+ * out_d[2r], out_d[2r+1] for row r, each reduced in the fixed order of
+ * hipnufft_dot_c64 (so row r equals hipnufft_dot_c64 on that row bit for
+ * bit).  ws_d: rows * HIPNUFFT_DOT_WS doubles or NULL; rows <= 65535.  The
+ * per-problem inner products of a batched CG (one reconstruction per row);
+ * the reference's CG reduces over the whole batch (extras/optim.py:872-877). */
+int hipnufft_dot_c64_rows(const void* a_d, const void* b_d, int64_t rows,
+                          int64_t n, int conj, double* out_d, double* ws_d,
+                          void* stream);
+
 /* Sorted-point introspection for bit-exact tests: copies the permutation
  * (sorted position -> original index, M int32), the bin offsets
- * (nbins+1 int32) and the spread lists (entries int32 and offsets
- * nbins+1 int32) into caller device buffers; any pointer may be NULL.
- * sizes[0..2] receive M, nbins, number of list entries. */
+ * (nkeys+1 int32) and the spread lists (entries int32 and offsets
+ * nkeys+1 int32) into caller device buffers; any pointer may be NULL.
+ * sizes[0..2] receive M, nkeys (point sets x tiles), number of list
+ * entries.  Keys are set * tiles + tile. */
 int hipnufft_get_sort(const hipnufft_plan* plan, int32_t* perm_d,
                       int32_t* bin_start_d, int32_t* list_d,
                       int32_t* list_start_d, int64_t* sizes);

@@ -182,6 +182,7 @@ void free_plan(hipnufft_plan_s* p) {
   }
   (void)hipFree(p->pt_idx);
   (void)hipFree(p->bin_start);
+  (void)hipFree(p->set_off);
   (void)hipFree(p->sub_tile);
   (void)hipFree(p->sub_begin);
   (void)hipFree(p->list_pts);
@@ -231,11 +232,42 @@ int ensure_ksorted(hipnufft_plan_s* p) {
   return ensure_kbuf(p, &p->ksorted, &p->ksorted_cap, "sorted k-space workspace");
 }
 
-// chunk size for B*C transforms: whole batches when they fit
-int chunk_of(const hipnufft_plan_s* p, int C, bool sense) {
-  int T = p->ntrans;
-  if (sense && T >= C) T = (T / C) * C;
-  return std::max(T, 1);
+// One launch chunk of a SENSE call: chunk transforms [t0, t0 + T) of the
+// B*C (batch-major), the point sets they use, and the element offset of the
+// chunk's k-space in the caller's array.
+struct Chunk {
+  int t0, T;
+  SetRange r;
+  int64_t kofs;
+};
+
+// One point set: every batch shares it; chunks of whole batches when they
+// fit ntrans (SENSE), otherwise slices of ntrans transforms.
+// Several sets: batch b uses set b (B == nsets) with its own C coils; chunks
+// of whole sets, and k-space is addressed per set ([k][c][M_k]).
+int make_chunks(const hipnufft_plan_s* p, int B, int C, bool sense, std::vector<Chunk>& out) {
+  out.clear();
+  const int TT = B * C;
+  if (p->nsets == 1) {
+    int T = p->ntrans;
+    if (sense && T >= C) T = (T / C) * C;
+    T = std::max(T, 1);
+    for (int t0 = 0; t0 < TT; t0 += T) {
+      const int Tc = std::min(T, TT - t0);
+      out.push_back(Chunk{t0, Tc, SetRange{0, 1, Tc}, (int64_t)t0 * p->M});
+    }
+    return 0;
+  }
+  if (B != p->nsets)
+    return set_error(HIPNUFFT_ERR_ARG, "a plan with several point sets needs one batch per set");
+  if (C > p->ntrans)
+    return set_error(HIPNUFFT_ERR_ARG, "a plan with several point sets needs ntrans >= coils");
+  const int spc = p->ntrans / C;  // sets per chunk
+  for (int k0 = 0; k0 < p->nsets; k0 += spc) {
+    const int k1 = std::min(p->nsets, k0 + spc);
+    out.push_back(Chunk{k0 * C, (k1 - k0) * C, SetRange{k0, k1, C}, 0});
+  }
+  return 0;
 }
 
 __global__ void k_zero_f32(float* x, int64_t n) {
@@ -248,9 +280,13 @@ __global__ void k_zero_f32(float* x, int64_t n) {
 // ----------------------------------------------------------------------------
 constexpr int kDotBlocks = HIPNUFFT_DOT_WS / 2;
 
+// row blockIdx.y of a[rows][n], b[rows][n]; partials part[row][gridDim.x]
 __global__ __launch_bounds__(kThreads) void k_dot_partial(const float2* a, const float2* b,
                                                           int64_t n, int conj, double2* part) {
   __shared__ double sr[kThreads], si[kThreads];
+  a += (int64_t)blockIdx.y * n;
+  b += (int64_t)blockIdx.y * n;
+  part += (int64_t)blockIdx.y * gridDim.x;
   double re = 0.0, im = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * kThreads) {
@@ -272,8 +308,11 @@ __global__ __launch_bounds__(kThreads) void k_dot_partial(const float2* a, const
   if (threadIdx.x == 0) part[blockIdx.x] = make_double2(sr[0], si[0]);
 }
 
+// row blockIdx.x: out[2 row], out[2 row + 1]
 __global__ __launch_bounds__(kThreads) void k_dot_final(const double2* part, int nb, double* out) {
   __shared__ double sr[kThreads], si[kThreads];
+  part += (int64_t)blockIdx.x * nb;
+  out += 2 * (int64_t)blockIdx.x;
   double re = 0.0, im = 0.0;
   for (int i = threadIdx.x; i < nb; i += kThreads) {
     re += part[i].x;
@@ -297,14 +336,17 @@ __global__ __launch_bounds__(kThreads) void k_dot_final(const double2* part, int
 
 }  // namespace
 
-int dot_c64(const float2* a, const float2* b, int64_t n, int conj, double* out, double* ws,
-            hipStream_t s) {
+int dot_c64(const float2* a, const float2* b, int64_t rows, int64_t n, int conj, double* out,
+            double* ws, hipStream_t s) {
+  if (rows <= 0) return 0;
+  if (rows > 65535) return set_error(HIPNUFFT_ERR_ARG, "too many dot rows (max 65535)");
   const int nb = (int)std::max<int64_t>(1, std::min<int64_t>(kDotBlocks, (n + kThreads - 1) / kThreads));
   double2* part = reinterpret_cast<double2*>(ws);
-  if (!part) HN_HIP(hipMallocAsync((void**)&part, sizeof(double2) * nb, s));
-  hipLaunchKernelGGL(k_dot_partial, dim3(nb), dim3(kThreads), 0, s, a, b, n, conj, part);
+  if (!part) HN_HIP(hipMallocAsync((void**)&part, sizeof(double2) * nb * rows, s));
+  hipLaunchKernelGGL(k_dot_partial, dim3(nb, (unsigned)rows), dim3(kThreads), 0, s, a, b, n, conj,
+                     part);
   HN_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kThreads), 0, s, part, nb, out);
+  hipLaunchKernelGGL(k_dot_final, dim3((unsigned)rows), dim3(kThreads), 0, s, part, nb, out);
   HN_LAUNCH_CHECK();
   if (!ws) HN_HIP(hipFreeAsync(part, s));
   return 0;
@@ -433,9 +475,18 @@ int hipnufft_setpts(hipnufft_plan* p, int64_t M, const float* x0, const float* x
   if (!p) return set_error(HIPNUFFT_ERR_ARG, "null plan");
   if (M < 0) return set_error(HIPNUFFT_ERR_ARG, "negative number of points");
   const float* xd[3] = {x0, x1, x2};
-  int rc = build_points(p, M, xd);
-  if (rc) return rc;
-  return 0;
+  return build_points(p, 1, &M, xd);
+}
+
+int hipnufft_setpts_sets(hipnufft_plan* p, int nsets, const int64_t* Ms, const float* x0,
+                         const float* x1, const float* x2) {
+  g_last_error.clear();
+  if (!p) return set_error(HIPNUFFT_ERR_ARG, "null plan");
+  if (nsets < 1 || !Ms) return set_error(HIPNUFFT_ERR_ARG, "need at least one point set");
+  if (nsets > 1 && p->sio)
+    return set_error(HIPNUFFT_ERR_ARG, "spread/interp-only plans take a single point set");
+  const float* xd[3] = {x0, x1, x2};
+  return build_points(p, nsets, Ms, xd);
 }
 
 int hipnufft_sense_op(hipnufft_plan* p, const void* img, const void* smaps, void* ksp, int B,
@@ -447,17 +498,23 @@ int hipnufft_sense_op(hipnufft_plan* p, const void* img, const void* smaps, void
   const float2* x = (const float2*)img;
   const float2* s = (const float2*)smaps;
   float2* y = (float2*)ksp;
-  const int TT = B * C;
   if (p->sio) {
     if (s) return set_error(HIPNUFFT_ERR_ARG, "spread/interp-only plans take no smaps");
-    return launch_interp(p, x, img_size(p), TT, y, nullptr, nullptr, scale, 0);
+    const int TT = B * C;
+    for (int t0 = 0; t0 < TT; t0 += p->ntrans) {
+      const int Tc = std::min(p->ntrans, TT - t0);
+      if ((rc = launch_interp(p, x + (int64_t)t0 * img_size(p), img_size(p), SetRange{0, 1, Tc},
+                              y + (int64_t)t0 * p->M, nullptr, nullptr, scale, 0)))
+        return rc;
+    }
+    return 0;
   }
-  const int T = chunk_of(p, C, s != nullptr);
-  for (int t0 = 0; t0 < TT; t0 += T) {
-    const int Tc = std::min(T, TT - t0);
-    if ((rc = fwd_to_grid(p, x, s, C, t0, Tc, 1.0f, p->sign2))) return rc;
-    if ((rc = launch_interp(p, p->grid, grid_size(p), Tc, y + (int64_t)t0 * p->M, nullptr,
-                            nullptr, scale, 0)))
+  std::vector<Chunk> chunks;
+  if ((rc = make_chunks(p, B, C, s != nullptr, chunks))) return rc;
+  const int sall = p->nsets > 1;
+  for (const Chunk& c : chunks) {
+    if ((rc = fwd_to_grid(p, x, s, C, c.t0, c.T, 1.0f, p->sign2, 0, sall))) return rc;
+    if ((rc = launch_interp(p, p->grid, grid_size(p), c.r, y + c.kofs, nullptr, nullptr, scale, 0)))
       return rc;
   }
   return 0;
@@ -472,25 +529,25 @@ int hipnufft_sense_adj(hipnufft_plan* p, const void* ksp, const void* smaps, con
   const float2* y = (const float2*)ksp;
   const float2* s = (const float2*)smaps;
   float2* x = (float2*)img;
-  const int TT = B * C;
   if ((rc = ensure_ksorted(p))) return rc;
   if (p->sio) {
     if (s) return set_error(HIPNUFFT_ERR_ARG, "spread/interp-only plans take no smaps");
     if (scale != 1.0f) return set_error(HIPNUFFT_ERR_ARG, "spread/interp-only adjoint has no scale");
+    const int TT = B * C;
     for (int t0 = 0; t0 < TT; t0 += p->ntrans) {
       const int Tc = std::min(p->ntrans, TT - t0);
-      if ((rc = launch_spread(p, y + (int64_t)t0 * p->M, density, Tc, x + (int64_t)t0 * img_size(p),
-                              img_size(p))))
+      if ((rc = launch_spread(p, y + (int64_t)t0 * p->M, density, SetRange{0, 1, Tc},
+                              x + (int64_t)t0 * img_size(p), img_size(p))))
         return rc;
     }
     return 0;
   }
-  const int T = chunk_of(p, C, s != nullptr);
-  for (int t0 = 0; t0 < TT; t0 += T) {
-    const int Tc = std::min(T, TT - t0);
-    if ((rc = launch_spread(p, y + (int64_t)t0 * p->M, density, Tc, p->grid, grid_size(p))))
-      return rc;
-    if ((rc = grid_to_img(p, x, s, C, t0, Tc, scale, -p->sign2))) return rc;
+  std::vector<Chunk> chunks;
+  if ((rc = make_chunks(p, B, C, s != nullptr, chunks))) return rc;
+  const int sall = p->nsets > 1;
+  for (const Chunk& c : chunks) {
+    if ((rc = launch_spread(p, y + c.kofs, density, c.r, p->grid, grid_size(p)))) return rc;
+    if ((rc = grid_to_img(p, x, s, C, c.t0, c.T, scale, -p->sign2, sall))) return rc;
   }
   return 0;
 }
@@ -507,30 +564,32 @@ int hipnufft_sense_dc(hipnufft_plan* p, const void* img, const void* smaps, cons
   const float2* s = (const float2*)smaps;
   const float2* y = (const float2*)obs;
   float2* g = (float2*)grad;
-  const int TT = B * C;
-  const int T = chunk_of(p, C, s != nullptr);
-  for (int t0 = 0; t0 < TT; t0 += T) {
-    const int Tc = std::min(T, TT - t0);
-    if ((rc = fwd_to_grid(p, x, s, C, t0, Tc, 1.0f, p->sign2))) return rc;
-    if ((rc = launch_interp(p, p->grid, grid_size(p), Tc, p->kwork, y + (int64_t)t0 * p->M, density,
-                            scale, 1)))
+  std::vector<Chunk> chunks;
+  if ((rc = make_chunks(p, B, C, s != nullptr, chunks))) return rc;
+  const int sall = p->nsets > 1;
+  for (const Chunk& c : chunks) {
+    if ((rc = fwd_to_grid(p, x, s, C, c.t0, c.T, 1.0f, p->sign2, 0, sall))) return rc;
+    if ((rc = launch_interp(p, p->grid, grid_size(p), c.r, p->kwork, y + c.kofs, density, scale,
+                            1)))
       return rc;
-    if ((rc = launch_spread_sorted(p, p->kwork, Tc, p->grid, grid_size(p)))) return rc;
-    if ((rc = grid_to_img(p, g, s, C, t0, Tc, scale, -p->sign2))) return rc;
+    if ((rc = launch_spread_sorted(p, p->kwork, c.r, p->grid, grid_size(p)))) return rc;
+    if ((rc = grid_to_img(p, g, s, C, c.t0, c.T, scale, -p->sign2, sall))) return rc;
   }
   return 0;
 }
 
 int hipnufft_execute(hipnufft_plan* p, const void* in, void* out) {
   if (!p) return set_error(HIPNUFFT_ERR_ARG, "null plan");
-  if (p->type == 2) return hipnufft_sense_op(p, in, nullptr, out, p->ntrans, 1, 1.0f);
-  return hipnufft_sense_adj(p, in, nullptr, nullptr, out, p->ntrans, 1, 1.0f);
+  const int B = p->nsets > 1 ? p->nsets : p->ntrans;
+  if (p->type == 2) return hipnufft_sense_op(p, in, nullptr, out, B, 1, 1.0f);
+  return hipnufft_sense_adj(p, in, nullptr, nullptr, out, B, 1, 1.0f);
 }
 
 int hipnufft_execute_adjoint(hipnufft_plan* p, const void* in, void* out) {
   if (!p) return set_error(HIPNUFFT_ERR_ARG, "null plan");
-  if (p->type == 2) return hipnufft_sense_adj(p, in, nullptr, nullptr, out, p->ntrans, 1, 1.0f);
-  return hipnufft_sense_op(p, in, nullptr, out, p->ntrans, 1, 1.0f);
+  const int B = p->nsets > 1 ? p->nsets : p->ntrans;
+  if (p->type == 2) return hipnufft_sense_adj(p, in, nullptr, nullptr, out, B, 1, 1.0f);
+  return hipnufft_sense_op(p, in, nullptr, out, B, 1, 1.0f);
 }
 
 int hipnufft_sense_grad_traj(hipnufft_plan* p, const void* img, const void* smaps, const void* dy,
@@ -544,20 +603,18 @@ int hipnufft_sense_grad_traj(hipnufft_plan* p, const void* img, const void* smap
   const float2* x = (const float2*)img;
   const float2* s = (const float2*)smaps;
   const float2* d = (const float2*)dy;
+  std::vector<Chunk> chunks;
+  if ((rc = make_chunks(p, B, C, s != nullptr, chunks))) return rc;
   const int64_t n = p->M * p->ndim;
   if (n > 0) {
     hipLaunchKernelGGL(k_zero_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, p->stream,
                        gtraj, n);
     HN_LAUNCH_CHECK();
   }
-  const int TT = B * C;
-  const int T = chunk_of(p, C, s != nullptr);
-  for (int t0 = 0; t0 < TT; t0 += T) {
-    const int Tc = std::min(T, TT - t0);
-    if ((rc = fwd_to_grid(p, x, s, C, t0, Tc, 1.0f, iflag, flags & 1))) return rc;
-    if ((rc = launch_grad_traj(p, p->grid, grid_size(p), Tc, d + (int64_t)t0 * p->M, scale,
-                               gtraj)))
-      return rc;
+  const int sall = p->nsets > 1;
+  for (const Chunk& c : chunks) {
+    if ((rc = fwd_to_grid(p, x, s, C, c.t0, c.T, 1.0f, iflag, flags & 1, sall))) return rc;
+    if ((rc = launch_grad_traj(p, p->grid, grid_size(p), c.r, d + c.kofs, scale, gtraj))) return rc;
   }
   return 0;
 }
@@ -566,7 +623,14 @@ int hipnufft_dot_c64(const void* a, const void* b, int64_t n, int conj, double* 
                      void* stream) {
   if (!a || !b || !out) return set_error(HIPNUFFT_ERR_ARG, "null pointer");
   if (n < 0) return set_error(HIPNUFFT_ERR_ARG, "negative length");
-  return dot_c64((const float2*)a, (const float2*)b, n, conj, out, ws, (hipStream_t)stream);
+  return dot_c64((const float2*)a, (const float2*)b, 1, n, conj, out, ws, (hipStream_t)stream);
+}
+
+int hipnufft_dot_c64_rows(const void* a, const void* b, int64_t rows, int64_t n, int conj,
+                          double* out, double* ws, void* stream) {
+  if (!a || !b || !out) return set_error(HIPNUFFT_ERR_ARG, "null pointer");
+  if (n < 0 || rows < 0) return set_error(HIPNUFFT_ERR_ARG, "negative size");
+  return dot_c64((const float2*)a, (const float2*)b, rows, n, conj, out, ws, (hipStream_t)stream);
 }
 
 int hipnufft_get_sort(const hipnufft_plan* p, int32_t* perm, int32_t* bin_start, int32_t* list,
@@ -574,19 +638,20 @@ int hipnufft_get_sort(const hipnufft_plan* p, int32_t* perm, int32_t* bin_start,
   int rc = need_pts(p);
   if (rc) return rc;
   hipStream_t s = p->stream;
+  const int64_t nkeys = p->nbins * p->nsets;
   if (perm && p->M)
     HN_HIP(hipMemcpyAsync(perm, p->pt_idx, sizeof(int32_t) * p->M, hipMemcpyDeviceToDevice, s));
   if (bin_start)
-    HN_HIP(hipMemcpyAsync(bin_start, p->bin_start, sizeof(int32_t) * (p->nbins + 1),
+    HN_HIP(hipMemcpyAsync(bin_start, p->bin_start, sizeof(int32_t) * (nkeys + 1),
                           hipMemcpyDeviceToDevice, s));
   if (list && p->nlist)
     HN_HIP(hipMemcpyAsync(list, p->list_pts, sizeof(int32_t) * p->nlist, hipMemcpyDeviceToDevice, s));
   if (list_start)
-    HN_HIP(hipMemcpyAsync(list_start, p->list_start, sizeof(int32_t) * (p->nbins + 1),
+    HN_HIP(hipMemcpyAsync(list_start, p->list_start, sizeof(int32_t) * (nkeys + 1),
                           hipMemcpyDeviceToDevice, s));
   if (sizes) {
     sizes[0] = p->M;
-    sizes[1] = p->nbins;
+    sizes[1] = nkeys;
     sizes[2] = p->nlist;
   }
   return 0;
@@ -609,7 +674,7 @@ int hipnufft_workspace_bytes(const hipnufft_plan* p, size_t* bytes) {
   if (!p || !bytes) return set_error(HIPNUFFT_ERR_ARG, "null pointer");
   size_t b = p->bytes;
   b += p->M_cap * (sizeof(int32_t) + p->ndim * (sizeof(int32_t) + sizeof(float)));
-  b += (p->list_cap + 2 * p->sub_cap + 2 * (p->nbins + 1)) * sizeof(int32_t);
+  b += (p->list_cap + 2 * p->sub_cap + 2 * p->keys_cap + p->set_off_cap) * sizeof(int32_t);
   b += (3 * p->items_cap + 3 * p->dense_cap) * sizeof(int32_t) + p->part_cap * sizeof(float2);
   *bytes = b;
   return 0;

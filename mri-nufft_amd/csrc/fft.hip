@@ -200,6 +200,7 @@ struct RowsArgs {
   const float2* smaps;
   int C, t0, T;
   int conj_smaps;  // forward: multiply by conj(smap) instead of smap
+  int smaps_all;   // smaps hold one map per transform ([B][C]) instead of [C]
   float2* grid;
   int64_t gstride;
   int M0, M1, M2;
@@ -224,7 +225,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_fwd(RowsArgs a) {
   const int tt = a.t0 + t;
   const int64_t N = (int64_t)rows * a.M2;
   const float2* src = a.img + (int64_t)(a.smaps ? tt / a.C : tt) * N;
-  const float2* sm = a.smaps ? a.smaps + (int64_t)(tt % a.C) * N : nullptr;
+  const float2* sm = a.smaps ? a.smaps + (int64_t)(a.smaps_all ? tt : tt % a.C) * N : nullptr;
   const int n = a.nf2;
 
   // loads first (overlapping), then the arithmetic and LDS stores
@@ -337,7 +338,8 @@ __global__ __launch_bounds__(kThreads) void k_rows_adj(RowsArgs a) {
     int lg = a.lgL, lp = LP;
     asm volatile("" : "+s"(lg), "+s"(lp));
     fft_lines<S>(buf, twl, a.ax, lg, lp);
-    const float2* sm = a.smaps ? a.smaps + (int64_t)(tt % a.C) * N + (int64_t)r * a.M2 : nullptr;
+    const float2* sm =
+        a.smaps ? a.smaps + (int64_t)(a.smaps_all ? tt : tt % a.C) * N + (int64_t)r * a.M2 : nullptr;
 #pragma unroll
     for (int j = 0; j < kEPT; ++j) {
       const int x = x0 + j * TPL;
@@ -507,10 +509,11 @@ RowsArgs rows_args(const hipnufft_plan_s* p) {
 int64_t grid_size(const hipnufft_plan_s* p) { return p->nf[0] * p->nf[1] * p->nf[2]; }
 
 int fwd_to_grid(const hipnufft_plan_s* p, const float2* img, const float2* smaps, int C, int t0,
-                int T, float scale, int sign, int conj_smaps) {
+                int T, float scale, int sign, int conj_smaps, int smaps_all) {
   if (T <= 0) return 0;
   RowsArgs a = rows_args(p);
   a.conj_smaps = conj_smaps;
+  a.smaps_all = smaps_all;
   a.img = img;
   a.smaps = smaps;
   a.C = C;
@@ -539,7 +542,7 @@ int fwd_to_grid(const hipnufft_plan_s* p, const float2* img, const float2* smaps
 }
 
 int grid_to_img(const hipnufft_plan_s* p, float2* img, const float2* smaps, int C, int t0, int T,
-                float scale, int sign) {
+                float scale, int sign, int smaps_all) {
   if (T <= 0) return 0;
   int rc;
   if (p->ndim == 3 && (rc = strided(p, 0, T, 0, (int)p->N[0], 0, sign, kStageSlabsAdj)))
@@ -550,6 +553,7 @@ int grid_to_img(const hipnufft_plan_s* p, float2* img, const float2* smaps, int 
   RowsArgs a = rows_args(p);
   a.img_out = img;
   a.smaps = smaps;
+  a.smaps_all = smaps_all;
   a.C = C;
   a.t0 = t0;
   a.T = T;

@@ -45,6 +45,10 @@ struct FFTAxis {
 
 // Geometry of the points / tiles for spreading and interpolation.  Arrays
 // are indexed by the D actual dimensions (0 = slowest).
+//
+// A plan holds nsets point sets (trajectories) of M_k points each, stored
+// back to back: set k owns original indices and sorted positions
+// [set_off[k], set_off[k+1]).  Bin / tile keys are set * nbins + tile.
 struct PtsGeom {
   int D = 0;
   int nf[3] = {1, 1, 1};
@@ -52,10 +56,27 @@ struct PtsGeom {
   int nt[3] = {1, 1, 1};
   int w = 1;
   float beta = 0.f;
-  int64_t M = 0;
+  int64_t M = 0;                                         // all sets
+  int nbins = 1;                                         // tiles per set
+  int nsets = 1;
+  const int32_t* set_off = nullptr;                      // nsets + 1
   const int32_t* cell[3] = {nullptr, nullptr, nullptr};  // sorted order
   const float* off[3] = {nullptr, nullptr, nullptr};     // sorted order
   const int32_t* idx = nullptr;                          // sorted -> original
+};
+
+// The point sets one launch covers and how its transforms map onto them:
+// sets [k0, k1), ts transforms per set; transform t_local of set k is
+// chunk transform (k - k0) * ts + t_local.  A plan with one set shared by
+// all transforms of a chunk uses {0, 1, T}.
+//
+// k-space (non-uniform) arrays are laid out per set as [ts][M_k], sets back
+// to back: element (k, t_local, original index i) sits at
+//   ts * set_off[k] + t_local * M_k + (i - set_off[k]).
+// Sorted-order workspaces are [t_local][M] over all sets (position p).
+struct SetRange {
+  int k0 = 0, k1 = 1;
+  int ts = 1;
 };
 
 // Kernel stages timed by the optional per-plan instrumentation.
@@ -90,8 +111,15 @@ struct hipnufft_plan_s {
   double beta = 0.0;
   int tile[3] = {1, 1, 1};    // 3-padded
   int nt[3] = {1, 1, 1};      // tiles per axis, 3-padded
-  int64_t nbins = 1;
+  int64_t nbins = 1;           // tiles per point set
   int spread_group = 1, interp_group = 1, max_sub = 1024;
+  // point sets (trajectories): set_off[k] (host and device, nsets + 1) and the
+  // first interp item / spread item / dense tile of each set (host, nsets + 1)
+  int nsets = 1;
+  std::vector<int64_t> set_off_h{0, 0};
+  int32_t* set_off = nullptr;
+  int64_t set_off_cap = 0;
+  std::vector<int64_t> sub_of_set{0, 0}, item_of_set{0, 0}, dense_of_set{0, 0};
   hipStream_t stream = nullptr;
 
   float* dec[3] = {nullptr, nullptr, nullptr};  // deconvolution per padded axis
@@ -103,19 +131,19 @@ struct hipnufft_plan_s {
   int32_t* pt_cell[3] = {nullptr, nullptr, nullptr};
   float* pt_off[3] = {nullptr, nullptr, nullptr};
   int32_t* pt_idx = nullptr;
-  int32_t* bin_start = nullptr;   // nbins + 1
+  int32_t* bin_start = nullptr;   // nsets * nbins + 1
   int32_t* sub_tile = nullptr;    // interp work items
   int32_t* sub_begin = nullptr;
   int64_t nsub = 0;
   int32_t* list_pts = nullptr;    // spread lists (sorted-point positions)
-  int32_t* list_start = nullptr;  // nbins + 1
+  int32_t* list_start = nullptr;  // nsets * nbins + 1
   int64_t nlist = 0;
-  int64_t M_cap = 0, list_cap = 0, sub_cap = 0;
+  int64_t M_cap = 0, list_cap = 0, sub_cap = 0, keys_cap = 0;
 
   // spread work items: dense tiles are split into items of at most
   // spread_max entries whose partial tiles are summed in a fixed order
   int spread_max = 4096;
-  int32_t* it_tile = nullptr;   // per item
+  int32_t* it_tile = nullptr;   // per item: key (set * nbins + tile)
   int32_t* it_begin = nullptr;
   int32_t* it_slot = nullptr;   // -1: the item writes the grid directly
   int64_t nitems = 0, items_cap = 0;
@@ -161,46 +189,61 @@ PtsGeom pts_geom(const hipnufft_plan_s* p);
 size_t rows_adj_lds(const FFTAxis& ax, int64_t m2);
 
 // ---- sort.hip -------------------------------------------------------------
-// Bins the points and builds the sorted arrays, interp work items and
-// spread lists of the plan.  Synchronises the plan stream.
-int build_points(hipnufft_plan_s* p, int64_t M, const float* const* xd);
+// Bins the points of nsets sets (set k: Ms[k] points, coordinates back to
+// back in xd) and builds the sorted arrays, interp work items and spread
+// lists of the plan.  Synchronises the plan stream.
+int build_points(hipnufft_plan_s* p, int nsets, const int64_t* Ms, const float* const* xd);
+
+// device helper: the set owning position (original index or sorted
+// position) i, by bisection of set_off[0..nsets]
+__device__ __forceinline__ int find_set(const int32_t* set_off, int nsets, int64_t i) {
+  int lo = 0, hi = nsets;  // set_off[lo] <= i < set_off[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (set_off[mid] <= i) lo = mid; else hi = mid;
+  }
+  return lo;
+}
 
 // ---- spread_interp.hip ---------------------------------------------------
-// Interpolation over transforms [0, T): out[t][idx] (original order).
+// Interpolation of the chunk's transforms (SetRange r) from the grids.
+// out / obs are k-space arrays laid out as SetRange describes.
 // mode 0: out = scale * interp
-// mode 1: out[t][q] = (scale * interp - obs[t][idx]) * (density ? density[idx] : 1)
-//         written at the sorted position q (input of launch_spread_sorted)
+// mode 1: out[t_local][p] = (scale * interp - obs) * (density ? density[i] : 1),
+//         written in sorted order (input of launch_spread_sorted)
 int launch_interp(const hipnufft_plan_s* p, const float2* grid,
-                  int64_t grid_stride, int T, float2* out, const float2* obs,
+                  int64_t grid_stride, SetRange r, float2* out, const float2* obs,
                   const float* density, float scale, int mode);
-// Spreading of in[t][idx] * (density ? density[idx] : 1) for t < T into the
-// grid (every cell of every transform written once; no pre-zeroing).  The
+// Spreading of in * (density ? density[i] : 1) (k-space layout of SetRange)
+// into the chunk's grids (every cell written once; no pre-zeroing).  The
 // strengths are first gathered into the plan's sorted-order workspace.
 int launch_spread(const hipnufft_plan_s* p, const float2* in,
-                  const float* density, int T, float2* grid,
+                  const float* density, SetRange r, float2* grid,
                   int64_t grid_stride);
 // Same, for strengths already in sorted-point order (density applied).
 int launch_spread_sorted(const hipnufft_plan_s* p, const float2* in_sorted,
-                         int T, float2* grid, int64_t grid_stride);
-// Trajectory gradient by the kernel derivative (type-2 direction).
+                         SetRange r, float2* grid, int64_t grid_stride);
+// Trajectory gradient by the kernel derivative (type-2 direction); dy has
+// the k-space layout of SetRange.
 int launch_grad_traj(const hipnufft_plan_s* p, const float2* grid,
-                     int64_t grid_stride, int T, const float2* dy,
+                     int64_t grid_stride, SetRange r, const float2* dy,
                      float scale, float* gtraj);
 
 // ---- fft.hip -------------------------------------------------------------
 // Forward chain image(s) -> fine grid (deconvolve, pad, all FFT passes).
-// Transform t of the chunk uses image (smaps ? (t0+t)/C : t0+t) and coil
-// (t0+t)%C.
+// Transform tt = t0 + t of the chunk uses image (smaps ? tt/C : tt) and map
+// smaps[smaps_all ? tt : tt % C].
 int fwd_to_grid(const hipnufft_plan_s* p, const float2* img,
                 const float2* smaps, int C, int t0, int T, float scale,
-                int sign, int conj_smaps = 0);
+                int sign, int conj_smaps = 0, int smaps_all = 0);
 // Adjoint chain fine grid -> image(s) (all FFT passes, crop, deconvolve and,
 // with smaps, conjugate coil combine).  With smaps the chunk [t0, t0+T)
-// covers batches t0/C ... ; accumulate adds into img instead of writing.
+// covers batches t0/C ... (partial batches are accumulated into img).
 int grid_to_img(const hipnufft_plan_s* p, float2* img, const float2* smaps,
-                int C, int t0, int T, float scale, int sign);
+                int C, int t0, int T, float scale, int sign, int smaps_all = 0);
 
-int dot_c64(const float2* a, const float2* b, int64_t n, int conj,
+// rows independent FP64 dots of a[rows][n] and b[rows][n] -> out[rows][2]
+int dot_c64(const float2* a, const float2* b, int64_t rows, int64_t n, int conj,
             double* out, double* ws, hipStream_t s);
 
 // Exclusive scan of n int32 (device), result in out (may alias in), total in

@@ -89,6 +89,9 @@ struct FoldArgs {
   int nfi[3];
   double halfw;
   int64_t M;
+  const int32_t* set_off;  // point sets: key = set * nbins + tile
+  int nsets;
+  uint32_t nbins;
   uint32_t* key;
   int32_t* val;
   int32_t* cell[3];
@@ -123,6 +126,7 @@ __global__ __launch_bounds__(kThreads) void k_fold_bin(FoldArgs a) {
     if (c0 >= a.nfi[d]) c0 -= a.nfi[d];
     a.cell[d][i] = c0;
   }
+  if (a.nsets > 1) key += (uint32_t)find_set(a.set_off, a.nsets, i) * a.nbins;
   a.key[i] = key;
   a.val[i] = (int32_t)i;
 }
@@ -332,6 +336,9 @@ struct ListArgs {
   int nt[3];
   int w;
   int64_t M;
+  const int32_t* set_off;
+  int nsets;
+  uint32_t nbins;
   const int32_t* eoff;
   int32_t* cnt;
   uint32_t* lkey;
@@ -358,13 +365,16 @@ __global__ __launch_bounds__(kThreads) void k_list_emit(ListArgs a) {
   int n[3] = {1, 1, 1};
   for (int d = 0; d < D; ++d) n[d] = axis_tiles(a.cell[d][p], a.w, a.nf[d], a.tile[d], t[d]);
   int e = a.eoff[p];
+  // sorted positions of set k are [set_off[k], set_off[k+1]) like its
+  // original indices (the set is the key's most significant part)
+  const uint32_t kb = a.nsets > 1 ? (uint32_t)find_set(a.set_off, a.nsets, p) * a.nbins : 0u;
   for (int i0 = 0; i0 < n[0]; ++i0)
     for (int i1 = 0; i1 < (D > 1 ? n[1] : 1); ++i1)
       for (int i2 = 0; i2 < (D > 2 ? n[2] : 1); ++i2) {
         uint32_t key = (uint32_t)t[0][i0];
         if (D > 1) key = key * (uint32_t)a.nt[1] + (uint32_t)t[1][i1];
         if (D > 2) key = key * (uint32_t)a.nt[2] + (uint32_t)t[2][i2];
-        a.lkey[e] = key;
+        a.lkey[e] = kb + key;
         a.lval[e] = (int32_t)p;
         ++e;
       }
@@ -424,23 +434,51 @@ int scan_exclusive(const int32_t* in, int32_t* out, int64_t n, hipStream_t s, in
 
 // Spread work items from the list offsets (see k_item_count) and the
 // partial-tile scratch for the dense tiles.
+// dst[k] = src[k * stride] for k < n (the first scanned entry of each set)
+__global__ void k_pick(const int32_t* src, int64_t stride, int n, int32_t* dst) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) dst[k] = src[(int64_t)k * stride];
+}
+
+// out[k] = exclusive-scan value at the first key of set k, out[nsets] = total
+static int per_set_offsets(const hipnufft_plan_s* p, const int32_t* scanned, int64_t total,
+                           std::vector<int64_t>& out) {
+  const int K = p->nsets;
+  out.assign(K + 1, 0);
+  out[K] = total;
+  if (K == 1) return 0;
+  int32_t* d = nullptr;
+  HN_HIP(hipMallocAsync((void**)&d, sizeof(int32_t) * K, p->stream));
+  hipLaunchKernelGGL(k_pick, dim3((unsigned)((K + 255) / 256)), dim3(256), 0, p->stream, scanned,
+                     p->nbins, K, d);
+  HN_LAUNCH_CHECK();
+  std::vector<int32_t> h(K);
+  HN_HIP(hipMemcpyAsync(h.data(), d, sizeof(int32_t) * K, hipMemcpyDeviceToHost, p->stream));
+  HN_HIP(hipStreamSynchronize(p->stream));
+  HN_HIP(hipFree(d));
+  for (int k = 0; k < K; ++k) out[k] = h[k];
+  return 0;
+}
+
 static int build_spread_items(hipnufft_plan_s* p) {
   hipStream_t s = p->stream;
-  const int64_t nbins = p->nbins;
+  const int64_t nkeys = p->nbins * p->nsets;
   const int S = p->spread_max;
   int rc;
   int32_t *cnt = nullptr, *dcnt = nullptr, *dflag = nullptr;
-  HN_HIP(hipMallocAsync((void**)&cnt, sizeof(int32_t) * nbins, s));
-  HN_HIP(hipMallocAsync((void**)&dcnt, sizeof(int32_t) * nbins, s));
-  HN_HIP(hipMallocAsync((void**)&dflag, sizeof(int32_t) * nbins, s));
-  const unsigned g = (unsigned)((nbins + kThreads - 1) / kThreads);
-  hipLaunchKernelGGL(k_item_count, dim3(g), dim3(kThreads), 0, s, p->list_start, nbins, S, cnt,
+  HN_HIP(hipMallocAsync((void**)&cnt, sizeof(int32_t) * nkeys, s));
+  HN_HIP(hipMallocAsync((void**)&dcnt, sizeof(int32_t) * nkeys, s));
+  HN_HIP(hipMallocAsync((void**)&dflag, sizeof(int32_t) * nkeys, s));
+  const unsigned g = (unsigned)((nkeys + kThreads - 1) / kThreads);
+  hipLaunchKernelGGL(k_item_count, dim3(g), dim3(kThreads), 0, s, p->list_start, nkeys, S, cnt,
                      dcnt, dflag);
   HN_LAUNCH_CHECK();
   int64_t nitems = 0, nslots = 0, ndense = 0;
-  if ((rc = scan_exclusive(cnt, cnt, nbins, s, &nitems))) return rc;
-  if ((rc = scan_exclusive(dcnt, dcnt, nbins, s, &nslots))) return rc;
-  if ((rc = scan_exclusive(dflag, dflag, nbins, s, &ndense))) return rc;
+  if ((rc = scan_exclusive(cnt, cnt, nkeys, s, &nitems))) return rc;
+  if ((rc = scan_exclusive(dcnt, dcnt, nkeys, s, &nslots))) return rc;
+  if ((rc = scan_exclusive(dflag, dflag, nkeys, s, &ndense))) return rc;
+  if ((rc = per_set_offsets(p, cnt, nitems, p->item_of_set))) return rc;
+  if ((rc = per_set_offsets(p, dflag, ndense, p->dense_of_set))) return rc;
   if (nitems > p->items_cap || !p->it_tile) {
     int64_t c0 = 0, c1 = 0, c2 = 0;
     if ((rc = grow(&p->it_tile, &c0, nitems))) return rc;
@@ -465,7 +503,7 @@ static int build_spread_items(hipnufft_plan_s* p) {
       return set_error(HIPNUFFT_ERR_ALLOC, "cannot allocate the spreading scratch");
     p->part_cap = part_need;
   }
-  hipLaunchKernelGGL(k_item_emit, dim3(g), dim3(kThreads), 0, s, p->list_start, nbins, S, cnt,
+  hipLaunchKernelGGL(k_item_emit, dim3(g), dim3(kThreads), 0, s, p->list_start, nkeys, S, cnt,
                      dcnt, dflag, p->it_tile, p->it_begin, p->it_slot, p->dn_tile, p->dn_slot0,
                      p->dn_count);
   HN_LAUNCH_CHECK();
@@ -478,14 +516,23 @@ static int build_spread_items(hipnufft_plan_s* p) {
   return 0;
 }
 
-int build_points(hipnufft_plan_s* p, int64_t M, const float* const* xd) {
+int build_points(hipnufft_plan_s* p, int nsets, const int64_t* Ms, const float* const* xd) {
   const int D = p->ndim;
   const int o = 3 - D;  // first real axis in the padded arrays
   hipStream_t s = p->stream;
   int rc;
+  if (nsets < 1 || !Ms) return set_error(HIPNUFFT_ERR_ARG, "need at least one point set");
+  std::vector<int64_t> soff(nsets + 1, 0);
+  for (int k = 0; k < nsets; ++k) {
+    if (Ms[k] < 0) return set_error(HIPNUFFT_ERR_ARG, "negative number of points");
+    soff[k + 1] = soff[k] + Ms[k];
+  }
+  const int64_t M = soff[nsets];
   if (M > INT32_MAX / 2) return set_error(HIPNUFFT_ERR_ARG, "too many points (max 2^30)");
+  if (p->nbins * nsets >= ((int64_t)1 << 31))
+    return set_error(HIPNUFFT_ERR_ARG, "too many point sets for this grid (sets x tiles >= 2^31)");
   for (int d = 0; d < D; ++d)
-    if (!xd[d]) return set_error(HIPNUFFT_ERR_ARG, "missing coordinate array");
+    if (!xd[d] && M > 0) return set_error(HIPNUFFT_ERR_ARG, "missing coordinate array");
 
   // persistent sorted arrays (one capacity for all of them)
   if (M > p->M_cap || !p->pt_idx) {
@@ -505,18 +552,40 @@ int build_points(hipnufft_plan_s* p, int64_t M, const float* const* xd) {
     HN_HIP(hipMalloc((void**)&p->pt_idx, sizeof(int32_t) * cap));
     p->M_cap = cap;
   }
+  // point-set offsets (host and device)
+  if (nsets + 1 > p->set_off_cap || !p->set_off) {
+    if (p->set_off) HN_HIP(hipFree(p->set_off));
+    p->set_off = nullptr;
+    HN_HIP(hipMalloc((void**)&p->set_off, sizeof(int32_t) * (nsets + 1)));
+    p->set_off_cap = nsets + 1;
+  }
+  {
+    std::vector<int32_t> so(soff.begin(), soff.end());
+    HN_HIP(hipMemcpyAsync(p->set_off, so.data(), sizeof(int32_t) * (nsets + 1),
+                          hipMemcpyHostToDevice, s));
+    HN_HIP(hipStreamSynchronize(s));  // so is a stack buffer
+  }
+  p->nsets = nsets;
+  p->set_off_h = soff;
   p->M = M;
   p->has_pts = 1;
 
-  const int64_t nbins = p->nbins;
-  if (!p->bin_start) HN_HIP(hipMalloc((void**)&p->bin_start, sizeof(int32_t) * (nbins + 1)));
-  if (!p->list_start) HN_HIP(hipMalloc((void**)&p->list_start, sizeof(int32_t) * (nbins + 1)));
+  const int64_t nkeys = p->nbins * nsets;
+  if (nkeys + 1 > p->keys_cap || !p->bin_start) {
+    if (p->bin_start) HN_HIP(hipFree(p->bin_start));
+    if (p->list_start) HN_HIP(hipFree(p->list_start));
+    p->bin_start = p->list_start = nullptr;
+    HN_HIP(hipMalloc((void**)&p->bin_start, sizeof(int32_t) * (nkeys + 1)));
+    HN_HIP(hipMalloc((void**)&p->list_start, sizeof(int32_t) * (nkeys + 1)));
+    p->keys_cap = nkeys + 1;
+  }
 
   if (M == 0) {
-    HN_HIP(hipMemsetAsync(p->bin_start, 0, sizeof(int32_t) * (nbins + 1), s));
-    HN_HIP(hipMemsetAsync(p->list_start, 0, sizeof(int32_t) * (nbins + 1), s));
+    HN_HIP(hipMemsetAsync(p->bin_start, 0, sizeof(int32_t) * (nkeys + 1), s));
+    HN_HIP(hipMemsetAsync(p->list_start, 0, sizeof(int32_t) * (nkeys + 1), s));
     p->nsub = 0;
     p->nlist = 0;
+    p->sub_of_set.assign(nsets + 1, 0);
     int rc0 = build_spread_items(p);  // empty tiles still write zeros
     if (rc0) return rc0;
     HN_HIP(hipStreamSynchronize(s));
@@ -551,6 +620,9 @@ int build_points(hipnufft_plan_s* p, int64_t M, const float* const* xd) {
   }
   fa.halfw = 0.5 * p->w;
   fa.M = M;
+  fa.set_off = p->set_off;
+  fa.nsets = nsets;
+  fa.nbins = (uint32_t)p->nbins;
   fa.key = ka;
   fa.val = va;
   const unsigned nb = (unsigned)((M + kThreads - 1) / kThreads);
@@ -559,12 +631,12 @@ int build_points(hipnufft_plan_s* p, int64_t M, const float* const* xd) {
   if (D == 3) hipLaunchKernelGGL(k_fold_bin<3>, dim3(nb), dim3(kThreads), 0, s, fa);
   HN_LAUNCH_CHECK();
 
-  const int bits = bits_for(nbins);
+  const int bits = bits_for(nkeys);
   if ((rc = radix_sort(ka, va, kb, vb, M, bits, s))) return rc;
 
   {
     const unsigned g = (unsigned)((M + 1 + kThreads - 1) / kThreads);
-    hipLaunchKernelGGL(k_bin_start, dim3(g), dim3(kThreads), 0, s, ka, M, nbins, p->bin_start);
+    hipLaunchKernelGGL(k_bin_start, dim3(g), dim3(kThreads), 0, s, ka, M, nkeys, p->bin_start);
     HN_LAUNCH_CHECK();
   }
   {
@@ -587,13 +659,14 @@ int build_points(hipnufft_plan_s* p, int64_t M, const float* const* xd) {
   // interp work items: split dense bins into chunks of max_sub points
   {
     int32_t* cnt = nullptr;
-    HN_HIP(hipMallocAsync((void**)&cnt, sizeof(int32_t) * nbins, s));
-    const unsigned g = (unsigned)((nbins + kThreads - 1) / kThreads);
-    hipLaunchKernelGGL(k_sub_count, dim3(g), dim3(kThreads), 0, s, p->bin_start, nbins, p->max_sub,
+    HN_HIP(hipMallocAsync((void**)&cnt, sizeof(int32_t) * nkeys, s));
+    const unsigned g = (unsigned)((nkeys + kThreads - 1) / kThreads);
+    hipLaunchKernelGGL(k_sub_count, dim3(g), dim3(kThreads), 0, s, p->bin_start, nkeys, p->max_sub,
                        cnt);
     HN_LAUNCH_CHECK();
     int64_t nsub = 0;
-    if ((rc = scan_exclusive(cnt, cnt, nbins, s, &nsub))) return rc;
+    if ((rc = scan_exclusive(cnt, cnt, nkeys, s, &nsub))) return rc;
+    if ((rc = per_set_offsets(p, cnt, nsub, p->sub_of_set))) return rc;
     if (nsub > p->sub_cap || !p->sub_tile) {
       int64_t c1 = 0, c2 = 0;
       if ((rc = grow(&p->sub_tile, &c1, nsub))) return rc;
@@ -601,7 +674,7 @@ int build_points(hipnufft_plan_s* p, int64_t M, const float* const* xd) {
       p->sub_cap = c1;
     }
     p->nsub = nsub;
-    hipLaunchKernelGGL(k_sub_emit, dim3(g), dim3(kThreads), 0, s, p->bin_start, cnt, nbins,
+    hipLaunchKernelGGL(k_sub_emit, dim3(g), dim3(kThreads), 0, s, p->bin_start, cnt, nkeys,
                        p->max_sub, p->sub_tile, p->sub_begin);
     HN_LAUNCH_CHECK();
     HN_HIP(hipFreeAsync(cnt, s));
@@ -618,6 +691,9 @@ int build_points(hipnufft_plan_s* p, int64_t M, const float* const* xd) {
     }
     la.w = p->w;
     la.M = M;
+    la.set_off = p->set_off;
+    la.nsets = nsets;
+    la.nbins = (uint32_t)p->nbins;
     la.cnt = vb;  // reuse temporaries: vb = counts, then offsets
     if (D == 1) hipLaunchKernelGGL(k_list_count<1>, dim3(nb), dim3(kThreads), 0, s, la);
     if (D == 2) hipLaunchKernelGGL(k_list_count<2>, dim3(nb), dim3(kThreads), 0, s, la);
@@ -644,7 +720,7 @@ int build_points(hipnufft_plan_s* p, int64_t M, const float* const* xd) {
     HN_LAUNCH_CHECK();
     if ((rc = radix_sort(lk0, p->list_pts, lk1, lv1, L, bits, s))) return rc;
     const unsigned g = (unsigned)((L + 1 + kThreads - 1) / kThreads);
-    hipLaunchKernelGGL(k_bin_start, dim3(g), dim3(kThreads), 0, s, lk0, L, nbins, p->list_start);
+    hipLaunchKernelGGL(k_bin_start, dim3(g), dim3(kThreads), 0, s, lk0, L, nkeys, p->list_start);
     HN_LAUNCH_CHECK();
     HN_HIP(hipFreeAsync(lk0, s));
     HN_HIP(hipFreeAsync(lk1, s));

@@ -53,32 +53,42 @@ struct SpreadArgs {
   float2* part;             // [slot][ntrans][tile cells]
   int ntrans;
   int smax;
-  const float2* in;
+  int item0, dn0;           // first spread item / dense tile of the launch
+  const float2* in;         // sorted order [t_local][M]
   int64_t in_stride;
-  const float* density;
   float2* grid;
   int64_t grid_stride;
-  int T, G;
+  int k0, ts;               // SetRange of the launch
+  int G;
 };
 
 struct InterpArgs {
   PtsGeom g;
-  const int32_t* sub_tile;
+  const int32_t* sub_tile;  // key (set * nbins + tile) per work item
   const int32_t* sub_begin;
   const int32_t* bin_start;
   int max_sub;
+  int sub0;                 // first work item of the launch
   const float2* grid;
   int64_t grid_stride;
   float2* out;
-  int64_t out_stride;
   const float2* obs;
   const float* density;
   float* gtraj;
   float dscale[3];  // radians -> cells factor per axis (gradient)
-  int T, G;
+  int k0, ts;       // SetRange of the launch
+  int tg0, G;       // transforms [tg0, tg0 + G) of each set (gradient groups)
   float scale;
   int mode;
 };
+
+// offset of (set k, transform t_local, original index i) in a k-space array
+// laid out per set as [ts][M_k]
+__device__ __forceinline__ int64_t ksp_index(const PtsGeom& g, int ts, int k, int t, int i) {
+  const int64_t o = g.set_off[k];
+  const int64_t mk = g.set_off[k + 1] - o;
+  return ts * o + t * mk + (i - o);
+}
 
 template <int D>
 __device__ __forceinline__ void tile_coords(int key, const int* nt, int* tc) {
@@ -113,11 +123,14 @@ __global__ __launch_bounds__(kThreads) void k_spread(SpreadArgs a) {
   extern __shared__ float smem[];
   const int W = a.g.w;
   const int PW = D + D * W + 2 * G;  // floats per staged point
-  const int item = blockIdx.x;
-  const int tile = a.it_tile[item];
+  const int item = a.item0 + blockIdx.x;
+  const int key = a.it_tile[item];
+  const int set = key / a.g.nbins;
+  const int tile = key - set * a.g.nbins;
   const int slot = a.it_slot[item];
-  const int t0 = blockIdx.y * G;
-  const int Gv = min(G, a.T - t0);
+  const int t0 = blockIdx.y * G;         // first transform of the set handled here
+  const int tb = (set - a.k0) * a.ts;    // chunk transform of the set's first
+  const int Gv = min(G, a.ts - t0);
   const float inv_hw = 2.0f / W, beta = a.g.beta;
 
   int tc[3], org[3], ext[3], td[3], nf[3];
@@ -158,7 +171,7 @@ __global__ __launch_bounds__(kThreads) void k_spread(SpreadArgs a) {
     for (int g = 0; g < G; ++g) acc[j][g] = make_float2(0.f, 0.f);
 
   const int e0 = a.it_begin[item];
-  const int e1 = min(e0 + a.smax, a.list_start[tile + 1]);
+  const int e1 = min(e0 + a.smax, a.list_start[key + 1]);
   for (int eb = e0; eb < e1; eb += kSpreadChunk) {
     const int n = min(kSpreadChunk, e1 - eb);
     if ((int)threadIdx.x < n) {
@@ -238,7 +251,7 @@ __global__ __launch_bounds__(kThreads) void k_spread(SpreadArgs a) {
       const int r = col * zl + z0 + j;
 #pragma unroll
       for (int g = 0; g < G; ++g)
-        if (g < Gv) a.part[((int64_t)slot * a.ntrans + t0 + g) * TC + r] = acc[j][g];
+        if (g < Gv) a.part[((int64_t)slot * a.ntrans + tb + t0 + g) * TC + r] = acc[j][g];
       continue;
     }
     int64_t gi = 0;
@@ -247,16 +260,18 @@ __global__ __launch_bounds__(kThreads) void k_spread(SpreadArgs a) {
     gi = gi * nf[DL] + org[DL] + z0 + j;
 #pragma unroll
     for (int g = 0; g < G; ++g)
-      if (g < Gv) a.grid[(int64_t)(t0 + g) * a.grid_stride + gi] = acc[j][g];
+      if (g < Gv) a.grid[(int64_t)(tb + t0 + g) * a.grid_stride + gi] = acc[j][g];
   }
 }
 
 // Sum the partial tiles of each dense tile in slot order (deterministic).
 template <int D>
 __global__ __launch_bounds__(kThreads) void k_spread_reduce(SpreadArgs a) {
-  const int j = blockIdx.x;
-  const int t = blockIdx.y;
-  const int tile = a.dn_tile[j], slot0 = a.dn_slot0[j], cnt = a.dn_count[j];
+  const int j = a.dn0 + blockIdx.x;
+  const int key = a.dn_tile[j], slot0 = a.dn_slot0[j], cnt = a.dn_count[j];
+  const int set = key / a.g.nbins;
+  const int tile = key - set * a.g.nbins;
+  const int t = (set - a.k0) * a.ts + blockIdx.y;  // chunk transform
   int tc[3], org[3], ext[3], td[3];
   tile_coords<D>(tile, a.g.nt, tc);
   int TC = 1;
@@ -362,14 +377,17 @@ __device__ __forceinline__ void point_setup(const InterpArgs& a, int p, const in
 template <int D, int W>
 __global__ __launch_bounds__(kThreads) void k_interp(InterpArgs a) {
   extern __shared__ float2 lds[];
-  const int sub = blockIdx.x;
-  const int tile = a.sub_tile[sub];
+  const int sub = a.sub0 + blockIdx.x;
+  const int key = a.sub_tile[sub];
+  const int set = key / a.g.nbins;
+  const int tile = key - set * a.g.nbins;
   const int pb = a.sub_begin[sub];
-  const int pe = min(pb + a.max_sub, a.bin_start[tile + 1]);
-  const int t0 = blockIdx.y * a.G;
-  const int G = min(a.G, a.T - t0);
+  const int pe = min(pb + a.max_sub, a.bin_start[key + 1]);
+  const int t0 = blockIdx.y * a.G;       // first transform of the set handled here
+  const int G = min(a.G, a.ts - t0);
+  const int tb = (set - a.k0) * a.ts;    // chunk transform of the set's first
   int rorg[3], R[3], RC;
-  stage_region<D, W>(a, lds, tile, t0, G, rorg, R, RC);
+  stage_region<D, W>(a, lds, tile, tb + t0, G, rorg, R, RC);
   __syncthreads();
 
   for (int p = pb + threadIdx.x; p < pe; p += kThreads) {
@@ -426,16 +444,16 @@ __global__ __launch_bounds__(kThreads) void k_interp(InterpArgs a) {
           sy += w0 * py;
         }
       }
-      const int64_t row = (int64_t)(t0 + gg) * a.out_stride;
+      const int64_t ko = ksp_index(a.g, a.ts, set, t0 + gg, idx);
       float2 r = make_float2(a.scale * sx, a.scale * sy);
       if (a.mode == 1) {
         // residual for the spreading that follows: written in sorted order
-        const float2 y = a.obs[row + idx];
+        const float2 y = a.obs[ko];
         r.x = (r.x - y.x) * dens;
         r.y = (r.y - y.y) * dens;
-        a.out[row + p] = r;
+        a.out[(int64_t)(t0 + gg) * a.g.M + p] = r;
       } else {
-        a.out[row + idx] = r;
+        a.out[ko] = r;
       }
     }
   }
@@ -445,13 +463,16 @@ __global__ __launch_bounds__(kThreads) void k_interp(InterpArgs a) {
 template <int D, int W>
 __global__ __launch_bounds__(kThreads) void k_grad_traj(InterpArgs a) {
   extern __shared__ float2 lds[];
-  const int sub = blockIdx.x;
-  const int tile = a.sub_tile[sub];
+  const int sub = a.sub0 + blockIdx.x;
+  const int key = a.sub_tile[sub];
+  const int set = key / a.g.nbins;
+  const int tile = key - set * a.g.nbins;
   const int pb = a.sub_begin[sub];
-  const int pe = min(pb + a.max_sub, a.bin_start[tile + 1]);
-  const int G = a.T;  // the whole chunk in one workgroup
+  const int pe = min(pb + a.max_sub, a.bin_start[key + 1]);
+  const int G = a.G;  // transforms [tg0, tg0 + G) of the set in one workgroup
+  const int tb = (set - a.k0) * a.ts + a.tg0;
   int rorg[3], R[3], RC;
-  stage_region<D, W>(a, lds, tile, 0, G, rorg, R, RC);
+  stage_region<D, W>(a, lds, tile, tb, G, rorg, R, RC);
   __syncthreads();
   const float inv_hw = 2.0f / W;
 
@@ -538,7 +559,7 @@ __global__ __launch_bounds__(kThreads) void k_grad_traj(InterpArgs a) {
         der[1] = d1;
         der[2] = d2;
       }
-      const float2 y = a.obs[(int64_t)gg * a.out_stride + idx];  // dy
+      const float2 y = a.obs[ksp_index(a.g, a.ts, set, a.tg0 + gg, idx)];  // dy
 #pragma unroll
       for (int d = 0; d < D; ++d) acc[d] += y.x * der[d].x + y.y * der[d].y;  // Re(conj(dy) der)
     }
@@ -593,6 +614,9 @@ PtsGeom pts_geom(const hipnufft_plan_s* p) {
   g.w = p->w;
   g.beta = (float)p->beta;
   g.M = p->M;
+  g.nbins = (int)p->nbins;
+  g.nsets = p->nsets;
+  g.set_off = p->set_off;
   g.idx = p->pt_idx;
   return g;
 }
@@ -603,47 +627,59 @@ static int tile_cells(const hipnufft_plan_s* p, int halo) {
   return c;
 }
 
+static int check_range(const hipnufft_plan_s* p, SetRange r) {
+  if (r.k0 < 0 || r.k1 > p->nsets || r.k0 >= r.k1 || r.ts < 1)
+    return set_error(HIPNUFFT_ERR_ARG, "invalid point-set range");
+  if ((int64_t)(r.k1 - r.k0) * r.ts > p->ntrans)
+    return set_error(HIPNUFFT_ERR_ARG, "chunk larger than ntrans");
+  return 0;
+}
+
 namespace {
 
-// ksorted[t][q] = in[t][idx[q]] * density[idx[q]]: one pass that reads the
-// caller's k-space once (runs of consecutive samples stay contiguous after
-// binning) and writes the spreading input in sorted-point order.
-__global__ __launch_bounds__(kThreads) void k_gather_sorted(const float2* in, const float* density,
-                                                            const int32_t* idx, int64_t M,
-                                                            float2* out) {
-  const int64_t q = (int64_t)blockIdx.x * kThreads + threadIdx.x;
-  if (q >= M) return;
-  const int64_t t = blockIdx.y;
-  const int32_t o = idx[q];
-  float2 v = in[t * M + o];
+// ksorted[t][q] = in[set k of q, t, idx[q]] * density[idx[q]] for the sorted
+// positions q of the launch's sets: one pass that reads the caller's k-space
+// once (runs of consecutive samples stay contiguous after binning) and
+// writes the spreading input in sorted-point order.
+__global__ __launch_bounds__(kThreads) void k_gather_sorted(PtsGeom g, int ts, int64_t q0,
+                                                            int64_t q1, const float2* in,
+                                                            const float* density, float2* out) {
+  const int64_t q = q0 + (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (q >= q1) return;
+  const int t = blockIdx.y;
+  const int k = find_set(g.set_off, g.nsets, q);
+  const int32_t o = g.idx[q];
+  float2 v = in[ksp_index(g, ts, k, t, o)];
   if (density) {
     const float d = density[o];
     v.x *= d;
     v.y *= d;
   }
-  out[t * M + q] = v;
+  out[(int64_t)t * g.M + q] = v;
 }
 
 }  // namespace
 
-int launch_spread(const hipnufft_plan_s* p, const float2* in, const float* density, int T,
+int launch_spread(const hipnufft_plan_s* p, const float2* in, const float* density, SetRange r,
                   float2* grid, int64_t grid_stride) {
-  if (T <= 0) return 0;
-  if (!p->ksorted || p->ksorted_cap < (int64_t)T * p->M)
+  int rc = check_range(p, r);
+  if (rc) return rc;
+  if (!p->ksorted || p->ksorted_cap < (int64_t)r.ts * p->M)
     return set_error(HIPNUFFT_ERR_ARG, "sorted k-space workspace missing");
-  if (p->M > 0) {
-    const dim3 g((unsigned)((p->M + kThreads - 1) / kThreads), (unsigned)T);
-    hipLaunchKernelGGL(k_gather_sorted, g, dim3(kThreads), 0, p->stream, in, density, p->pt_idx,
-                       p->M, p->ksorted);
+  const int64_t q0 = p->set_off_h[r.k0], q1 = p->set_off_h[r.k1];
+  if (q1 > q0) {
+    const dim3 g((unsigned)((q1 - q0 + kThreads - 1) / kThreads), (unsigned)r.ts);
+    hipLaunchKernelGGL(k_gather_sorted, g, dim3(kThreads), 0, p->stream, pts_geom(p), r.ts, q0,
+                       q1, in, density, p->ksorted);
     HN_LAUNCH_CHECK();
   }
-  return launch_spread_sorted(p, p->ksorted, T, grid, grid_stride);
+  return launch_spread_sorted(p, p->ksorted, r, grid, grid_stride);
 }
 
-int launch_spread_sorted(const hipnufft_plan_s* p, const float2* in, int T, float2* grid,
+int launch_spread_sorted(const hipnufft_plan_s* p, const float2* in, SetRange r, float2* grid,
                          int64_t grid_stride) {
-  if (T <= 0) return 0;
-  const float* density = nullptr;
+  int rc = check_range(p, r);
+  if (rc) return rc;
   SpreadArgs a{};
   a.g = pts_geom(p);
   a.list_pts = p->list_pts;
@@ -659,11 +695,14 @@ int launch_spread_sorted(const hipnufft_plan_s* p, const float2* in, int T, floa
   a.smax = p->spread_max;
   a.in = in;
   a.in_stride = p->M;
-  a.density = density;
   a.grid = grid;
   a.grid_stride = grid_stride;
-  a.T = T;
-  if (T > p->ntrans) return set_error(HIPNUFFT_ERR_ARG, "spread chunk larger than ntrans");
+  a.k0 = r.k0;
+  a.ts = r.ts;
+  a.item0 = (int)p->item_of_set[r.k0];
+  a.dn0 = (int)p->dense_of_set[r.k0];
+  const int64_t nitems = p->item_of_set[r.k1] - p->item_of_set[r.k0];
+  const int64_t ndense = p->dense_of_set[r.k1] - p->dense_of_set[r.k0];
   {  // every (column, kZ-cell segment) of a tile needs its own thread
     const int zl = p->tile[2];
     const int ncol = tile_cells(p, 0) / zl;
@@ -672,15 +711,16 @@ int launch_spread_sorted(const hipnufft_plan_s* p, const float2* in, int T, floa
   }
   int G = p->spread_group;
   G = G >= 16 ? 16 : G >= 8 ? 8 : G >= 4 ? 4 : G >= 2 ? 2 : 1;
-  while (G > 1 && G / 2 >= T) G /= 2;
+  while (G > 1 && G / 2 >= r.ts) G /= 2;
   auto lds_of = [&](int g) {
     return sizeof(float) * (size_t)kSpreadChunk * (p->ndim * p->w + p->ndim + 2 * g);
   };
   while (G > 1 && lds_of(G) > kMaxLds) G /= 2;
   a.G = G;
   const size_t lds = lds_of(G);
-  const dim3 grd((unsigned)p->nitems, (unsigned)((T + G - 1) / G));
-  StageScope timer(p, kStageSpread);
+  if (nitems > 0) {
+    const dim3 grd((unsigned)nitems, (unsigned)((r.ts + G - 1) / G));
+    StageScope timer(p, kStageSpread);
 #define HN_SPREAD(DD)                                                                        \
   switch (G) {                                                                               \
     case 16: hipLaunchKernelGGL((k_spread<DD, 16>), grd, dim3(kThreads), lds, p->stream, a); break; \
@@ -689,13 +729,14 @@ int launch_spread_sorted(const hipnufft_plan_s* p, const float2* in, int T, floa
     case 2: hipLaunchKernelGGL((k_spread<DD, 2>), grd, dim3(kThreads), lds, p->stream, a); break; \
     default: hipLaunchKernelGGL((k_spread<DD, 1>), grd, dim3(kThreads), lds, p->stream, a); break; \
   }
-  if (p->ndim == 1) { HN_SPREAD(1) }
-  else if (p->ndim == 2) { HN_SPREAD(2) }
-  else { HN_SPREAD(3) }
+    if (p->ndim == 1) { HN_SPREAD(1) }
+    else if (p->ndim == 2) { HN_SPREAD(2) }
+    else { HN_SPREAD(3) }
 #undef HN_SPREAD
-  HN_LAUNCH_CHECK();
-  if (p->ndense > 0) {
-    const dim3 rg((unsigned)p->ndense, (unsigned)T);
+    HN_LAUNCH_CHECK();
+  }
+  if (ndense > 0) {
+    const dim3 rg((unsigned)ndense, (unsigned)r.ts);
     if (p->ndim == 1) hipLaunchKernelGGL(k_spread_reduce<1>, rg, dim3(kThreads), 0, p->stream, a);
     if (p->ndim == 2) hipLaunchKernelGGL(k_spread_reduce<2>, rg, dim3(kThreads), 0, p->stream, a);
     if (p->ndim == 3) hipLaunchKernelGGL(k_spread_reduce<3>, rg, dim3(kThreads), 0, p->stream, a);
@@ -704,31 +745,35 @@ int launch_spread_sorted(const hipnufft_plan_s* p, const float2* in, int T, floa
   return 0;
 }
 
-int launch_interp(const hipnufft_plan_s* p, const float2* grid, int64_t grid_stride, int T,
+int launch_interp(const hipnufft_plan_s* p, const float2* grid, int64_t grid_stride, SetRange r,
                   float2* out, const float2* obs, const float* density, float scale, int mode) {
-  if (T <= 0 || p->nsub == 0) return 0;
+  int rc = check_range(p, r);
+  if (rc) return rc;
+  const int64_t nsub = p->sub_of_set[r.k1] - p->sub_of_set[r.k0];
+  if (nsub <= 0) return 0;
   InterpArgs a{};
   a.g = pts_geom(p);
   a.sub_tile = p->sub_tile;
   a.sub_begin = p->sub_begin;
   a.bin_start = p->bin_start;
   a.max_sub = p->max_sub;
+  a.sub0 = (int)p->sub_of_set[r.k0];
   a.grid = grid;
   a.grid_stride = grid_stride;
   a.out = out;
-  a.out_stride = p->M;
   a.obs = obs;
   a.density = density;
-  a.T = T;
+  a.k0 = r.k0;
+  a.ts = r.ts;
   a.scale = scale;
   a.mode = mode;
   const size_t cell_bytes = sizeof(float2) * (size_t)tile_cells(p, p->w);
   if (cell_bytes > kMaxLds) return set_error(HIPNUFFT_ERR_ARG, "interp tile too large for LDS");
-  int G = p->interp_group;
+  int G = std::min(p->interp_group, r.ts);
   while (G > 1 && cell_bytes * G > kMaxLds) --G;
   a.G = G;
   const size_t lds = cell_bytes * G;
-  const dim3 grd((unsigned)p->nsub, (unsigned)((T + G - 1) / G));
+  const dim3 grd((unsigned)nsub, (unsigned)((r.ts + G - 1) / G));
   StageScope timer(p, kStageInterp);
   HN_DIM_SWITCH(p->ndim, HN_WIDTH_SWITCH(p->w, {
     hipLaunchKernelGGL((k_interp<DD, WW>), grd, dim3(kThreads), lds, p->stream, a);
@@ -737,36 +782,41 @@ int launch_interp(const hipnufft_plan_s* p, const float2* grid, int64_t grid_str
   return 0;
 }
 
-int launch_grad_traj(const hipnufft_plan_s* p, const float2* grid, int64_t grid_stride, int T,
-                     const float2* dy, float scale, float* gtraj) {
-  if (T <= 0 || p->nsub == 0) return 0;
+int launch_grad_traj(const hipnufft_plan_s* p, const float2* grid, int64_t grid_stride,
+                     SetRange r, const float2* dy, float scale, float* gtraj) {
+  int rc = check_range(p, r);
+  if (rc) return rc;
+  const int64_t nsub = p->sub_of_set[r.k1] - p->sub_of_set[r.k0];
+  if (nsub <= 0) return 0;
   InterpArgs a{};
   a.g = pts_geom(p);
   a.sub_tile = p->sub_tile;
   a.sub_begin = p->sub_begin;
   a.bin_start = p->bin_start;
   a.max_sub = p->max_sub;
+  a.sub0 = (int)p->sub_of_set[r.k0];
   a.grid = grid;
   a.grid_stride = grid_stride;
-  a.out_stride = p->M;
   a.obs = dy;
   a.gtraj = gtraj;
   a.scale = scale;
+  a.k0 = r.k0;
+  a.ts = r.ts;
   const int o = 3 - p->ndim;
   for (int d = 0; d < p->ndim; ++d) a.dscale[d] = (float)((double)p->nf[o + d] / (2.0 * M_PI));
   const size_t cell_bytes = sizeof(float2) * (size_t)tile_cells(p, p->w);
-  // process the chunk in groups that fit LDS; each launch accumulates
-  int G = (int)(kMaxLds / cell_bytes);
+  // each workgroup handles the transforms of its set that fit LDS; the
+  // launches accumulate into gtraj in stream order
+  const int G = (int)(kMaxLds / cell_bytes);
   if (G < 1) return set_error(HIPNUFFT_ERR_ARG, "interp tile too large for LDS");
-  for (int t0 = 0; t0 < T; t0 += G) {
-    const int Tg = min(G, T - t0);
-    a.grid = grid + (int64_t)t0 * grid_stride;
-    a.obs = dy + (int64_t)t0 * p->M;
-    a.T = Tg;
+  for (int tg0 = 0; tg0 < r.ts; tg0 += G) {
+    const int Tg = std::min(G, r.ts - tg0);
+    a.tg0 = tg0;
+    a.G = Tg;
     const size_t lds = cell_bytes * Tg;
     StageScope timer(p, kStageGradTraj);
     HN_DIM_SWITCH(p->ndim, HN_WIDTH_SWITCH(p->w, {
-      hipLaunchKernelGGL((k_grad_traj<DD, WW>), dim3((unsigned)p->nsub), dim3(kThreads), lds,
+      hipLaunchKernelGGL((k_grad_traj<DD, WW>), dim3((unsigned)nsub), dim3(kThreads), lds,
                          p->stream, a);
     }));
     HN_LAUNCH_CHECK();

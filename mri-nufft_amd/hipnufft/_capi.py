@@ -57,6 +57,7 @@ _SIGNATURES = {
     "hipnufft_plan_create": [_I, _I, ctypes.POINTER(_I64), _I, _I, ctypes.c_double,
                              ctypes.POINTER(Opts), _P, ctypes.POINTER(_P)],
     "hipnufft_setpts": [_P, _I64, _P, _P, _P],
+    "hipnufft_setpts_sets": [_P, _I, ctypes.POINTER(_I64), _P, _P, _P],
     "hipnufft_execute": [_P, _P, _P],
     "hipnufft_execute_adjoint": [_P, _P, _P],
     "hipnufft_sense_op": [_P, _P, _P, _P, _I, _I, _F],
@@ -64,6 +65,7 @@ _SIGNATURES = {
     "hipnufft_sense_dc": [_P, _P, _P, _P, _P, _P, _I, _I, _F],
     "hipnufft_sense_grad_traj": [_P, _P, _P, _P, _I, _I, _F, _I, _I, _P],
     "hipnufft_dot_c64": [_P, _P, _I64, _I, _P, _P, _P],
+    "hipnufft_dot_c64_rows": [_P, _P, _I64, _I64, _I, _P, _P, _P],
     "hipnufft_get_sort": [_P, _P, _P, _P, _P, ctypes.POINTER(_I64)],
     "hipnufft_plan_info": [_P, ctypes.POINTER(_I64), ctypes.POINTER(_I),
                            ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_I),
@@ -187,6 +189,19 @@ class RawPlan:
         check(lib().hipnufft_setpts(self._h, c.shape[0], *args), "setpts")
         self.n_samples = int(c.shape[0])
 
+    def setpts_sets(self, coords_list):
+        """Several point sets at once: a list of (M_k, ndim) float32 radians
+        tensors on the device (one trajectory each)."""
+        self._sync_stream()
+        Ms = [int(c.shape[0]) for c in coords_list]
+        c = torch.cat([x.to(torch.float32).reshape(-1, self.ndim) for x in coords_list], 0)
+        cols = [c[:, d].contiguous() for d in range(self.ndim)]
+        args = [ptr(x) for x in cols] + [None] * (3 - self.ndim)
+        arr = (_I64 * len(Ms))(*Ms)
+        check(lib().hipnufft_setpts_sets(self._h, len(Ms), arr, *args), "setpts_sets")
+        self.n_samples = int(c.shape[0])
+        self.set_sizes = Ms
+
     def info(self):
         nf = (_I64 * 3)()
         w = _I()
@@ -285,4 +300,20 @@ def dot_c64(a, b, conj=True, out=None, ws=None):
         out = torch.empty(2, dtype=torch.float64, device=a.device)
     check(lib().hipnufft_dot_c64(ptr(a), ptr(b), a.numel(), 1 if conj else 0, ptr(out), ptr(ws),
                                  current_stream()), "dot")
+    return out
+
+
+def dot_c64_rows(a, b, conj=True, out=None, ws=None):
+    """Row-wise FP64 dots of (R, n) complex64 device tensors -> (R, 2) float64
+    [re, im] per row (row r equals ``dot_c64(a[r], b[r])`` bit for bit);
+    ``ws`` is an optional (R * DOT_WS,) float64 workspace."""
+    a = a.resolve_conj().resolve_neg().contiguous()
+    b = b.resolve_conj().resolve_neg().contiguous()
+    if a.shape != b.shape or a.ndim != 2:
+        raise ValueError("dot_c64_rows: need two (rows, n) tensors of the same shape")
+    R, n = a.shape
+    if out is None:
+        out = torch.empty((R, 2), dtype=torch.float64, device=a.device)
+    check(lib().hipnufft_dot_c64_rows(ptr(a), ptr(b), R, n, 1 if conj else 0, ptr(out), ptr(ws),
+                                      current_stream()), "dot_rows")
     return out

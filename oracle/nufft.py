@@ -29,7 +29,8 @@ FP64 NDFT of ``oracle.ndft`` (relative L2 ~1e-6 at eps = 1e-6, the accuracy
 the FINUFFT paper states); exact adjointness; ``phihat`` against a
 closed-form Fourier transform check on a Gaussian-like kernel and a brute
 trapezoid sum; the stable sort against ``numpy.argsort(kind="stable")`` and a
-pure-Python counting sort.
+pure-Python counting sort; the spread lists against brute-force enumeration;
+several point sets sorted together against each set sorted alone.
 """
 
 from __future__ import annotations
@@ -149,8 +150,18 @@ def tiles_per_dim(nf, tile):
     return [(n + t - 1) // t for n, t in zip(nf, tile)]
 
 
-def bin_keys(omega, nf, tile):
-    """C-order tile key of the tile containing each point."""
+def set_ids(M, set_sizes=None):
+    """Point-set index of each of the M points (sets stored back to back)."""
+    if set_sizes is None:
+        return np.zeros(M, dtype=np.int64)
+    ids = np.repeat(np.arange(len(set_sizes), dtype=np.int64), np.asarray(set_sizes))
+    assert ids.shape[0] == M
+    return ids
+
+
+def bin_keys(omega, nf, tile, set_sizes=None):
+    """C-order tile key of the tile containing each point; with several point
+    sets (``set_sizes``, sets back to back) the key is set * ntiles + tile."""
     omega = np.asarray(omega, dtype=np.float32)
     nt = tiles_per_dim(nf, tile)
     key = np.zeros(omega.shape[0], dtype=np.int64)
@@ -158,7 +169,7 @@ def bin_keys(omega, nf, tile):
         g = fold(omega[:, d], nf[d])
         b = np.floor(g * (1.0 / tile[d])).astype(np.int64)
         key = key * nt[d] + b
-    return key
+    return key + set_ids(omega.shape[0], set_sizes) * int(np.prod(nt))
 
 
 def stable_sort(keys):
@@ -182,17 +193,20 @@ def footprint_tiles(g, nf, tile, w):
     return out
 
 
-def spread_lists(omega, nf, tile, w):
+def spread_lists(omega, nf, tile, w, set_sizes=None):
     """Tile -> sorted-point lists with duplicates (for gather spreading).
 
     Returns (list_keys, list_points): for every (point, touched tile) pair
-    generated in sorted-point order, the tile key and the sorted position of
-    the point, then stably sorted by tile key.
+    generated in sorted-point order, the tile key (set * ntiles + tile with
+    several point sets) and the sorted position of the point, then stably
+    sorted by key.
     """
     omega = np.asarray(omega, dtype=np.float32)
-    keys = bin_keys(omega, nf, tile)
+    keys = bin_keys(omega, nf, tile, set_sizes)
     perm = stable_sort(keys)
     nt = tiles_per_dim(nf, tile)
+    sid = set_ids(omega.shape[0], set_sizes)[perm]
+    ntiles = int(np.prod(nt))
     g = np.stack([fold(omega[:, d], nf[d]) for d in range(len(nf))], 1)[perm]
     lk, lp = [], []
     for p in range(len(perm)):
@@ -201,7 +215,7 @@ def spread_lists(omega, nf, tile, w):
         for d in range(len(nf)):
             combos = [c * nt[d] + t for c in combos for t in per[d]]
         for c in combos:
-            lk.append(c)
+            lk.append(int(sid[p]) * ntiles + c)
             lp.append(p)
     lk = np.asarray(lk, dtype=np.int64)
     lp = np.asarray(lp, dtype=np.int64)

@@ -152,3 +152,33 @@ def test_fold_edges():
     assert g[0] == pytest.approx(192.0, abs=1e-4) and g[1] == pytest.approx(192.0, abs=1e-4)
     off = nufft.footprint_offset(g, 7)
     assert np.all(off >= 2.5) and np.all(off <= 3.5)
+
+
+def test_point_sets_partition_sort():
+    """Several point sets sorted together: set k occupies sorted positions
+    [off_k, off_k+1), and within it the order and bins are exactly those of
+    set k sorted alone (shifted by off_k); the lists split the same way."""
+    shape, tile, w = (16, 12), (16, 8), 7
+    nf = (32, 24)
+    sizes = [37, 0, 120, 5]
+    rng = np.random.default_rng(5)
+    pts = [(rng.uniform(-0.5, 0.5, (m, 2)) * 2 * np.pi).astype(np.float32) for m in sizes]
+    om = np.concatenate(pts, 0)
+    off = np.concatenate([[0], np.cumsum(sizes)])
+    nbins = int(np.prod(nufft.tiles_per_dim(nf, tile)))
+    keys = nufft.bin_keys(om, nf, tile, sizes)
+    perm = nufft.stable_sort(keys)
+    lk, lp = nufft.spread_lists(om, nf, tile, w, sizes)
+    for k, m in enumerate(sizes):
+        seg = perm[off[k]:off[k + 1]]
+        assert np.all((seg >= off[k]) & (seg < off[k + 1]))
+        if m == 0:
+            continue
+        k1 = nufft.bin_keys(pts[k], nf, tile)
+        p1 = nufft.stable_sort(k1)
+        np.testing.assert_array_equal(seg - off[k], p1)
+        np.testing.assert_array_equal(keys[seg] - k * nbins, k1[p1])
+        l1k, l1p = nufft.spread_lists(pts[k], nf, tile, w)
+        sel = (lk >= k * nbins) & (lk < (k + 1) * nbins)
+        np.testing.assert_array_equal(lk[sel] - k * nbins, l1k)
+        np.testing.assert_array_equal(lp[sel] - off[k], l1p)
