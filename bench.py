@@ -168,6 +168,14 @@ def main():
                     help="run one plain device copy kernel (float32 multiply by 1) over this "
                          "many MiB before the workload, as a known-size reference for HBM "
                          "byte counters")
+    ap.add_argument("--serving", type=int, default=0, metavar="N",
+                    help="instead of the operator benchmark, run the serving workload "
+                         "(N mixed 2D/3D CG requests) served vs plain per-call")
+    ap.add_argument("--serving-rate", type=float, default=20.0,
+                    help="Poisson arrival rate (requests/s) of the serving workload")
+    ap.add_argument("--autotune", action="store_true",
+                    help="choose tile / group / split options with the autotuner "
+                         "(tuning time is reported as part of plan setup)")
     ap.add_argument("--plan-opt", action="append", default=[], metavar="KEY=INT",
                     help="plan tuning option passed to the operator (spread_group, "
                          "interp_group, max_subproblem, spread_max)")
@@ -181,6 +189,17 @@ def main():
     import torch
 
     import hipnufft
+
+    if args.serving > 0:
+        from hipnufft.serving import benchmark
+
+        res = benchmark(args.serving, args.serving_rate, log=log)
+        print(json.dumps({"metric": "served reconstructions/s (10-iteration CG, mixed 2D/3D)",
+                          "value": round(res["saturation"]["served"]["requests_per_s"], 3),
+                          "unit": "requests/s", "n_gpus": 1, "higher_is_better": True,
+                          "data": "synthetic (seeded trajectories, images, coil maps)",
+                          "serving": res}), flush=True)
+        return
     from hipnufft.trajectories import (ellipsoid_phantom, gaussian_smaps,
                                        initialize_3D_phyllotaxis_radial)
 
@@ -203,7 +222,11 @@ def main():
     dev = torch.device("cuda", 0)
     t = time.perf_counter()
     plan_opts = {k: int(v) for k, v in (o.split("=", 1) for o in args.plan_opt)}
+    if args.autotune:
+        plan_opts["autotune"] = True
     op = hipnufft.get_operator("hipnufft")(traj, shape, n_coils=C, smaps=smaps_np, **plan_opts)
+    if op.tuning is not None:
+        log(f"autotune: {op.tuning}")
     torch.cuda.synchronize()
     t_plan = time.perf_counter() - t
     log(f"operator ready in {t_plan:.2f}s; n_trans={op.n_trans}; info={op.raw_op.plan.info()}")
@@ -291,7 +314,9 @@ def main():
         "data": "synthetic (seeded phyllotaxis trajectory, ellipsoid phantom, Gaussian coil maps)",
         "config": {"workload": "3D phyllotaxis radial 4096x512 (2,097,152 pts), 192^3, "
                                "16 coils SENSE, eps 1e-6, sigma 2",
-                   "n_trans": T, "plan_setup_s": round(t_plan, 3), "plan_opts": plan_opts},
+                   "n_trans": T, "plan_setup_s": round(t_plan, 3), "plan_opts": plan_opts,
+                   "tuned": None if op.tuning is None else
+                   {k: v for k, v in op.tuning.items()}},
         "cg_latency_ms": cg,
         "accuracy": acc,
         "stages": stage_summary,

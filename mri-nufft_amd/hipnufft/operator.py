@@ -74,7 +74,9 @@ class MRIHipNUFFT(FourierOperatorBase, _ToggleGradPlanMixin):
     smaps_cached; verbose; squeeze_dims (default False); n_trans (None: all
     B*C transforms per launch within a 24 GiB workspace budget);
     async_transfer; and kwargs eps (1e-6), upsampfac (2.0), gpu_device_id,
-    tile, spread_group, interp_group, max_subproblem.
+    tile, spread_group, interp_group, max_subproblem, spread_max, and
+    autotune (time candidate plan options on this trajectory and keep the
+    fastest; explicit options win).
     """
 
     backend = "hipnufft"
@@ -104,6 +106,9 @@ class MRIHipNUFFT(FourierOperatorBase, _ToggleGradPlanMixin):
             max_subproblem=int(kwargs.pop("max_subproblem", 0)),
             spread_max=int(kwargs.pop("spread_max", 0)),
         )
+        explicit = {k for k in ("tile", "spread_group", "interp_group", "max_subproblem",
+                                "spread_max") if kwargs.get(k) not in (None, 0)}
+        do_tune = bool(kwargs.pop("autotune", False))
         for k in ("gpu_method", "gpu_sort", "gpu_kerevalmeth", "gpu_spreadinterponly"):
             kwargs.pop(k, None)
         if kwargs:
@@ -115,6 +120,15 @@ class MRIHipNUFFT(FourierOperatorBase, _ToggleGradPlanMixin):
             self.dtype = np.float32
             self._explicit_n_trans = n_trans is not None
             self.n_trans = self._choose_n_trans(n_trans)
+            self.tuning = None
+            if do_tune and not self._plan_opts["spreadinterponly"]:
+                from .autotune import autotune, plan_options
+
+                self.tuning = autotune(self._samples, self.shape, n_trans=self.n_trans,
+                                       eps=self._eps, upsampfac=self._plan_opts["upsampfac"])
+                for k, v in plan_options(self.tuning).items():
+                    if k not in explicit:
+                        self._plan_opts[k] = v
             self.raw_op = RawHipPlan(self._samples, self.shape, self.n_trans, self._eps,
                                      **self._plan_opts)
             if isinstance(density, (np.ndarray, torch.Tensor)):
@@ -272,16 +286,21 @@ class MRIHipNUFFT(FourierOperatorBase, _ToggleGradPlanMixin):
         return image
 
     # --------------------------------------------------------------- extras
-    def get_lipschitz_cst(self, max_iter=10, **kwargs):
-        """Power method on A^H D A without coils (``cufinufft.py:1239-1279``)."""
+    def get_lipschitz_cst(self, max_iter=10, seed=None, **kwargs):
+        """Power method on A^H D A without coils (``cufinufft.py:1239-1279``).
+
+        The start vector is unseeded random as in the reference; ``seed``
+        makes it reproducible (used by the serving loop so that a request
+        gives the same result alone and among others)."""
         n_coils, n_batchs, smaps, sq = self.n_coils, self.n_batchs, self._smaps, self.squeeze_dims
         self._smaps = None
         self.n_coils = 1
         self.n_batchs = 1
         self.squeeze_dims = True
+        rnd = np.random.random if seed is None else np.random.default_rng(seed).random
         try:
-            x = 1j * np.random.random(self.shape).astype(self.cpx_dtype, copy=False)
-            x += np.random.random(self.shape).astype(self.cpx_dtype, copy=False)
+            x = 1j * rnd(self.shape).astype(self.cpx_dtype, copy=False)
+            x += rnd(self.shape).astype(self.cpx_dtype, copy=False)
             xt = torch.from_numpy(x).to(self.device)
             L, _ = power_method(max_iter, self,
                                 norm_func=lambda v: torch.linalg.vector_norm(v.reshape(-1)), x=xt)

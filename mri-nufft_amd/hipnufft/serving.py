@@ -1,0 +1,448 @@
+"""Continuous-batching reconstruction server on one GPU.
+
+Requests (a trajectory, observed k-space, coil maps and a CG budget) arrive
+over time.  The server keeps a set of slots; each slot runs one request's CG
+on its own HIP stream (default priority), one iteration per scheduling
+round, so the work of every admitted request overlaps on the device.  A
+request is admitted as soon as a slot is free and its device memory fits the
+HBM budget, also while other requests are mid-solve; finished slots are
+released at once.  Plans (sorted trajectories), their Lipschitz constants
+and workspaces stay resident in an LRU cache keyed by the trajectory, so a
+repeated protocol pays for its sort once; coil maps are uploaded per request
+into the resident plan.
+
+Every CG iteration after the first two is a replay of a HIP graph captured
+per slot, and the stopping test is a device flag (as in
+``pinv_solver(optim="cg", graph=True)``), so a request gives the same result
+alone and among others: same kernels, same summation order, and a Lipschitz
+constant computed from a seeded start vector.
+
+The reference has no serving layer; each reconstruction there is a
+``pinv_solver`` call on a freshly built operator (``base.py:667-690``,
+``extras/optim.py:801-902``).
+"""
+
+from __future__ import annotations
+
+import math
+import time
+from collections import OrderedDict, deque
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _capi
+from ._array import proper_trajectory
+from .operator import MRIHipNUFFT
+
+
+@dataclass
+class ReconRequest:
+    """One reconstruction: CG on ``A^H A x = A^H y`` for a SENSE operator."""
+
+    samples: np.ndarray            # (M, d) in [-0.5, 0.5) or radians
+    shape: tuple
+    kspace: np.ndarray             # (C, M) complex64 observations
+    smaps: np.ndarray              # (C, *shape) complex64
+    max_iter: int = 10
+    tol: float = 1e-4
+    traj_key: object = None        # requests with equal keys share a resident plan
+    arrival: float = 0.0           # seconds after the start of the run
+    rid: int = -1
+
+
+@dataclass
+class ReconResult:
+    rid: int
+    image: np.ndarray
+    arrival: float
+    admitted: float
+    finished: float
+
+    @property
+    def latency(self):
+        return self.finished - self.arrival
+
+
+def estimate_bytes(shape, n_samples, n_coils, upsampfac=2.0):
+    """Device memory of one request's plan, data and CG state (bytes)."""
+    N = float(np.prod(shape))
+    nf = float(np.prod([max(int(upsampfac * n), 16) for n in shape]))
+    d = len(shape)
+    grid = n_coils * nf * 8
+    pts = n_samples * (4 + 8 * d) + 4.5 * n_samples * 4 * (2 if d == 3 else 1)
+    ksp = 3 * n_coils * n_samples * 8          # observations + two sorted workspaces
+    img = (n_coils + 6) * N * 8                 # coil maps + CG vectors
+    return int(grid + pts + ksp + img + (8 << 20))
+
+
+class _Plan:
+    """A resident operator for one trajectory (plan, sorted points, L)."""
+
+    def __init__(self, key, samples, shape, n_coils, stream, seed):
+        with torch.cuda.stream(stream):
+            self.op = MRIHipNUFFT(samples, shape, n_coils=n_coils, smaps=None,
+                                  squeeze_dims=False)
+            self.op._smaps = torch.zeros((n_coils, *shape), dtype=torch.complex64,
+                                         device=self.op.device)
+            self.lipschitz = float(self.op.get_lipschitz_cst(seed=seed))
+        self.key = key
+        self.bytes = estimate_bytes(shape, self.op.n_samples, n_coils)
+        self.in_use = False
+
+
+class _CGJob:
+    """One request's CG, advanced one iteration at a time on a slot stream."""
+
+    def __init__(self, plan, y, max_iter, tol, stream):
+        self.plan = plan
+        self.op = plan.op
+        self.stream = stream
+        self.max_iter = int(max_iter)
+        self.done_iters = 0
+        dev = self.op.device
+        shape = self.op.img_full_shape
+        self.shape = shape
+        inv_l = 1.0 / plan.lipschitz
+        with torch.cuda.stream(stream):
+            self.y = y
+            image = torch.zeros(shape, dtype=torch.complex64, device=dev)
+            grad = self.op.data_consistency(image, y).reshape(shape)
+            vel = grad / plan.lipschitz
+            image = image - vel
+            self.st = dict(image=image.contiguous(), grad=grad.contiguous(), vel=vel.contiguous(),
+                           done=torch.zeros((), dtype=torch.bool, device=dev))
+            self.bufs = dict(nrm=torch.empty(2, dtype=torch.float64, device=dev),
+                             num=torch.empty(2, dtype=torch.float64, device=dev),
+                             den=torch.empty(2, dtype=torch.float64, device=dev),
+                             ws=torch.empty(_capi.DOT_WS, dtype=torch.float64, device=dev),
+                             zero=torch.zeros((), dtype=torch.float64, device=dev))
+        self.inv_l = inv_l
+        self.tol2 = float(tol) ** 2
+        self.graph = None
+        self.event = None
+
+    def _step(self):
+        st, bufs, shape = self.st, self.bufs, self.shape
+        gn = self.op.data_consistency(st["image"], self.y).reshape(shape)
+        ws = bufs["ws"]
+        _capi.dot_c64(gn.reshape(-1), gn.reshape(-1), True, out=bufs["nrm"], ws=ws)
+        diff = gn - st["grad"]
+        _capi.dot_c64(gn.reshape(-1), diff.reshape(-1), False, out=bufs["num"], ws=ws)
+        _capi.dot_c64(st["grad"].reshape(-1), st["grad"].reshape(-1), False, out=bufs["den"],
+                      ws=ws)
+        stop = st["done"] | (bufs["nrm"][0] <= self.tol2)
+        n_re, n_im = bufs["num"][0], bufs["num"][1]
+        d_re, d_im = bufs["den"][0], bufs["den"][1]
+        dd = d_re * d_re + d_im * d_im
+        b_re = (n_re * d_re + n_im * d_im) / dd
+        b_im = (n_im * d_re - n_re * d_im) / dd
+        pos = (b_re > 0) | ((b_re == 0) & (b_im > 0))
+        b_re = torch.where(pos, b_re, bufs["zero"])
+        b_im = torch.where(pos, b_im, bufs["zero"])
+        beta = torch.complex(b_re, b_im).to(torch.complex64)
+        vel_new = gn + beta * st["vel"]
+        img_new = st["image"] - vel_new * self.inv_l
+        st["image"].copy_(torch.where(stop, st["image"], img_new))
+        st["vel"].copy_(torch.where(stop, st["vel"], vel_new))
+        st["grad"].copy_(torch.where(stop, st["grad"], gn))
+        st["done"].copy_(stop)
+
+    @property
+    def enqueued_all(self):
+        return self.done_iters >= self.max_iter
+
+    def advance(self, use_graph=True):
+        """Enqueue the next iteration (no host synchronisation)."""
+        if self.enqueued_all:
+            return
+        with torch.cuda.stream(self.stream):
+            if self.graph is not None:
+                self.graph.replay()
+            else:
+                self._step()
+                if use_graph and self.done_iters == 1 and self.max_iter > 2:
+                    # the first two iterations ran eagerly (all lazily created
+                    # workspaces exist); capture the third on this stream
+                    # without a device-wide synchronisation, then replay it
+                    g = torch.cuda.CUDAGraph()
+                    g.capture_begin()
+                    try:
+                        self._step()
+                    finally:
+                        g.capture_end()
+                    self.graph = g
+                    self.done_iters += 1
+                    self.graph.replay()
+        self.done_iters += 1
+        if self.enqueued_all:
+            self.event = torch.cuda.Event()
+            self.event.record(self.stream)
+
+    def finished(self):
+        return self.event is not None and self.event.query()
+
+    def result(self):
+        with torch.cuda.stream(self.stream):
+            return self.st["image"].reshape(self.op.shape).cpu().numpy()
+
+
+class ReconServer:
+    """Slot-based continuous batching under an HBM budget (see module doc)."""
+
+    def __init__(self, hbm_budget_bytes=64 << 30, max_slots=8, use_graph=True, seed=1234):
+        if not MRIHipNUFFT.available:
+            raise RuntimeError("hipnufft is not available (library or GPU missing)")
+        self.budget = int(hbm_budget_bytes)
+        self.max_slots = int(max_slots)
+        self.use_graph = bool(use_graph)
+        self.seed = int(seed)
+        self.streams = [torch.cuda.Stream() for _ in range(self.max_slots)]
+        self.free_streams = list(range(self.max_slots))
+        self.plans = OrderedDict()   # key -> _Plan (LRU order)
+        self.active = []             # (request, job, stream index, admitted time)
+        self.waiting = deque()
+        self.results = {}
+        self._next_id = 0
+        self.stats = dict(admitted=0, plan_hits=0, plan_builds=0, evictions=0)
+
+    # ------------------------------------------------------------- memory
+    def _resident_bytes(self):
+        return sum(p.bytes for p in self.plans.values())
+
+    def _make_room(self, need):
+        """Evict idle plans (LRU) until ``need`` more bytes fit; False if they cannot."""
+        while self._resident_bytes() + need > self.budget:
+            idle = [k for k, p in self.plans.items() if not p.in_use]
+            if not idle:
+                return False
+            del self.plans[idle[0]]
+            self.stats["evictions"] += 1
+        return True
+
+    # ----------------------------------------------------------- requests
+    def submit(self, req: ReconRequest):
+        if req.rid < 0:
+            req.rid = self._next_id
+        self._next_id = max(self._next_id, req.rid) + 1
+        self.waiting.append(req)
+        return req.rid
+
+    def _key(self, req):
+        if req.traj_key is not None:
+            return (req.traj_key, tuple(req.shape), req.kspace.shape[0])
+        return (id(req), tuple(req.shape), req.kspace.shape[0])
+
+    def _try_admit(self, req, now):
+        if not self.free_streams:
+            return False
+        key = self._key(req)
+        C = int(req.kspace.shape[0])
+        plan = self.plans.get(key)
+        if plan is not None and plan.in_use:
+            plan = None
+            key = (key, req.rid)  # a second resident copy for a concurrent request
+        if plan is None:
+            need = estimate_bytes(req.shape, req.kspace.shape[1], C)
+            if need > self.budget:
+                raise ValueError(f"request {req.rid} needs {need} bytes, over the HBM budget")
+            if not self._make_room(need):
+                return False
+        si = self.free_streams.pop()
+        stream = self.streams[si]
+        if plan is None:
+            traj = proper_trajectory(req.samples, normalize="pi")
+            plan = _Plan(key, traj, tuple(req.shape), C, stream, self.seed)
+            self.plans[key] = plan
+            self.stats["plan_builds"] += 1
+        else:
+            self.plans.move_to_end(key)
+            self.stats["plan_hits"] += 1
+        plan.in_use = True
+        dev = plan.op.device
+        with torch.cuda.stream(stream):
+            plan.op._smaps.copy_(torch.from_numpy(np.ascontiguousarray(req.smaps)).to(dev))
+            y = torch.from_numpy(np.ascontiguousarray(req.kspace, dtype=np.complex64)).to(dev)
+            y = y.reshape(1, C, -1)
+            job = _CGJob(plan, y, req.max_iter, req.tol, stream)
+        self.active.append((req, job, si, now))
+        self.stats["admitted"] += 1
+        return True
+
+    def _retire(self, now):
+        keep = []
+        for req, job, si, adm in self.active:
+            if job.finished():
+                img = job.result()
+                job.plan.in_use = False
+                self.free_streams.append(si)
+                self.results[req.rid] = ReconResult(req.rid, img, req.arrival, adm, now)
+            else:
+                keep.append((req, job, si, adm))
+        self.active = keep
+
+    def run(self, requests, realtime=True):
+        """Serve ``requests`` (sorted by ``arrival``) to completion.
+
+        With ``realtime`` requests become visible at their arrival times
+        (wall clock from the call); otherwise all are waiting at once.
+        Returns {rid: ReconResult}.
+        """
+        pending = deque(sorted(requests, key=lambda r: r.arrival))
+        t0 = time.perf_counter()
+        while pending or self.waiting or self.active:
+            now = time.perf_counter() - t0
+            while pending and (not realtime or pending[0].arrival <= now):
+                self.submit(pending.popleft())
+            while self.waiting and self._try_admit(self.waiting[0], now):
+                self.waiting.popleft()
+            for _, job, _, _ in self.active:
+                job.advance(self.use_graph)
+            self._retire(time.perf_counter() - t0)
+            if not self.active and not self.waiting and pending and realtime:
+                time.sleep(max(0.0, min(pending[0].arrival - (time.perf_counter() - t0), 0.05)))
+            elif self.active and all(j.enqueued_all for _, j, _, _ in self.active):
+                time.sleep(0.0002)  # everything is queued on the device: poll gently
+        torch.cuda.synchronize()
+        return self.results
+
+
+def serve_plain(requests, realtime=True, graph=False):
+    """The per-call baseline: requests one at a time in arrival order, each a
+    fresh operator and ``pinv_solver(optim="cg")`` (as a reference user would)."""
+    out = {}
+    t0 = time.perf_counter()
+    for req in sorted(requests, key=lambda r: r.arrival):
+        if realtime:
+            wait = req.arrival - (time.perf_counter() - t0)
+            if wait > 0:
+                time.sleep(wait)
+        adm = time.perf_counter() - t0
+        op = MRIHipNUFFT(req.samples, tuple(req.shape), n_coils=req.kspace.shape[0],
+                         smaps=req.smaps, squeeze_dims=False)
+        y = torch.from_numpy(np.ascontiguousarray(req.kspace, dtype=np.complex64)).to(op.device)
+        img = op.pinv_solver(y.reshape(1, req.kspace.shape[0], -1), optim="cg",
+                             max_iter=req.max_iter, tol=req.tol, graph=graph)
+        img = img.reshape(req.shape).cpu().numpy()
+        out[req.rid] = ReconResult(req.rid, img, req.arrival, adm, time.perf_counter() - t0)
+    return out
+
+
+# ---------------------------------------------------------------------------
+# the serving workload: three request kinds, Poisson arrivals
+# ---------------------------------------------------------------------------
+KINDS = ("radial2d", "spiral2d", "stars3d")
+
+
+def make_request(kind, rng, protocol_choices=True):
+    """A seeded request of one kind (trajectory, phantom data, coil maps).
+
+    radial2d: 256^2, Nc in 64..256 (steps of 64) x Ns 512, 1..32 coils;
+    spiral2d: 384^2, Nc in 32..96 (steps of 32) x Ns 4096, 16 coils;
+    stars3d:  128^3 stack of stars, Nc 256 x Ns 256 x 128 partitions, 8 coils.
+    The k-space is the SENSE forward of a random image (so the solve has a
+    known answer); requests of one protocol share a trajectory key.
+    """
+    from .trajectories import gaussian_smaps, initialize_2D_radial, initialize_2D_spiral, stack
+
+    if kind == "radial2d":
+        Nc = int(rng.choice([64, 128, 192, 256])) if protocol_choices else int(rng.integers(64, 257))
+        C = int(rng.integers(1, 33))
+        shape = (256, 256)
+        traj = initialize_2D_radial(Nc, 512).reshape(-1, 2)
+        key = ("radial2d", Nc)
+    elif kind == "spiral2d":
+        Nc = int(rng.choice([32, 64, 96])) if protocol_choices else int(rng.integers(32, 97))
+        C = 16
+        shape = (384, 384)
+        traj = initialize_2D_spiral(Nc, 4096).reshape(-1, 2)
+        key = ("spiral2d", Nc)
+    elif kind == "stars3d":
+        C = 8
+        shape = (128, 128, 128)
+        traj = stack(initialize_2D_radial(256, 256), 128).reshape(-1, 3)
+        key = ("stars3d",)
+    else:
+        raise ValueError(kind)
+    traj = traj.astype(np.float32)
+    smaps = gaussian_smaps(C, shape, seed=int(rng.integers(1 << 30)))
+    img = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64)
+    return traj, shape, C, smaps, img, key
+
+
+def synth_kspace(traj, shape, C, smaps, img, op=None):
+    """Observed k-space of a request (SENSE forward on the device); ``op`` is
+    an operator of the same trajectory to reuse (its maps are replaced)."""
+    if op is None:
+        op = MRIHipNUFFT(traj, shape, n_coils=C, smaps=None, squeeze_dims=False)
+    op._smaps = torch.from_numpy(np.ascontiguousarray(smaps)).to(op.device)
+    y = op.op(torch.from_numpy(img).to(op.device)).reshape(C, -1)
+    return y.cpu().numpy(), op
+
+
+def workload(n_requests, rate_hz, seed=0, kinds=KINDS, max_iter=10):
+    """Seeded Poisson arrivals of requests drawn uniformly from ``kinds``."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    reqs = []
+    ops = {}
+    for i in range(n_requests):
+        t += float(rng.exponential(1.0 / rate_hz)) if rate_hz > 0 else 0.0
+        kind = kinds[int(rng.integers(len(kinds)))]
+        traj, shape, C, smaps, img, key = make_request(kind, rng)
+        y, ops[(key, C)] = synth_kspace(traj, shape, C, smaps, img, ops.get((key, C)))
+        reqs.append(ReconRequest(traj, shape, y, smaps, max_iter=max_iter, traj_key=key,
+                                 arrival=t, rid=i))
+    ops.clear()
+    torch.cuda.empty_cache()
+    return reqs
+
+
+def summarize(results, requests=None):
+    lat = np.array([r.latency for r in results.values()])
+    span = max(r.finished for r in results.values()) - min(r.arrival for r in results.values())
+    return dict(n=len(lat), p50_ms=float(np.percentile(lat, 50) * 1e3),
+                p99_ms=float(np.percentile(lat, 99) * 1e3),
+                requests_per_s=float(len(lat) / span) if span > 0 else math.inf)
+
+
+def benchmark(n_requests=24, rate_hz=20.0, seed=0, max_slots=8, hbm_budget_bytes=96 << 30,
+              log=print):
+    """Served vs plain per-call on the same seeded request trace.
+
+    Two passes per form: ``saturation`` (every request waiting at t=0: the
+    sustainable reconstructions/s) and ``poisson`` (arrivals at ``rate_hz``:
+    p50/p99 latency including queueing).  Also checks every served result
+    against the plain one (relative L2; they differ only through the
+    Lipschitz start vector, seeded in the server and random in the plain
+    path).
+    """
+    reqs = workload(n_requests, rate_hz, seed=seed)
+    kinds = {}
+    for r in reqs:
+        kinds[r.traj_key[0]] = kinds.get(r.traj_key[0], 0) + 1
+    log(f"workload: {n_requests} requests {kinds}")
+    out = dict(n_requests=n_requests, rate_hz=rate_hz, kinds=kinds, max_slots=max_slots,
+               hbm_budget_gib=hbm_budget_bytes / 2 ** 30)
+    # warm both paths (library, kernels, allocator) on a few requests
+    ReconServer(hbm_budget_bytes, max_slots).run(reqs[:3], realtime=False)
+    serve_plain(reqs[:3], realtime=False)
+    torch.cuda.synchronize()
+    res = {}
+    for mode, realtime in (("saturation", False), ("poisson", True)):
+        srv = ReconServer(hbm_budget_bytes, max_slots)
+        served = srv.run(reqs, realtime=realtime)
+        torch.cuda.synchronize()
+        plain = serve_plain(reqs, realtime=realtime)
+        torch.cuda.synchronize()
+        res[mode] = dict(served=summarize(served), plain=summarize(plain), server=dict(srv.stats))
+        log(f"{mode}: served {res[mode]['served']} plain {res[mode]['plain']}")
+        if mode == "saturation":
+            errs = [float(np.linalg.norm(served[i].image - plain[i].image)
+                          / np.linalg.norm(plain[i].image)) for i in served]
+            out["max_rel_l2_served_vs_plain"] = max(errs)
+    out.update(res)
+    s, p = res["saturation"]["served"], res["saturation"]["plain"]
+    out["throughput_ratio"] = s["requests_per_s"] / p["requests_per_s"]
+    return out

@@ -204,3 +204,16 @@ def test_register_into_mrinufft(tmp_path, monkeypatch):
     assert avail == hipnufft.MRIHipNUFFT.available
     assert OwnBase.interfaces["hipnufft"] == own_before
     assert hipnufft.register_into_mrinufft() is True  # idempotent
+
+
+def test_autotune_candidates_respect_kernel_limits():
+    from hipnufft.autotune import tile_candidates, tile_ok
+
+    assert tile_ok((8, 8, 16), 7) and not tile_ok((8, 16, 16), 7)   # 512 segments
+    assert not tile_ok((16, 16, 16), 7)
+    assert tile_ok((32, 32), 7, interp_group=4) and not tile_ok((32, 32), 7, interp_group=6)
+    for nd in (1, 2, 3):
+        for w in (4, 7, 10):
+            cands = tile_candidates(nd, w)
+            assert cands and all(len(c) == nd and tile_ok(c, w) for c in cands)
+            assert all(t & (t - 1) == 0 for c in cands for t in c)  # powers of two
