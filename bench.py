@@ -114,6 +114,32 @@ def stage_bytes(stage, op, T):
     return per_coil[stage] * T + once.get(stage, 0)
 
 
+def path_bytes(op, C):
+    """Algorithmic bytes of one SENSE op+adj_op over C coils: image read and
+    written, coil maps read twice, k-space written and read, coordinates read
+    twice, and one write + one read of every coil's fine grid per direction."""
+    N = math.prod(op.shape)
+    G = math.prod(op.raw_op.plan.info()["nf"])
+    M = op.n_samples
+    d = len(op.shape)
+    return 2 * N * 8 + 2 * C * N * 8 + 2 * C * M * 8 + 2 * M * d * 4 + 4 * C * G * 8
+
+
+def measured_traffic(stage):
+    """HBM bytes per launch of ``stage`` from the committed rocprofv3 PMC
+    summary of this workload (FETCH_SIZE, WRITE_SIZE scaled by a plain copy
+    kernel), or None."""
+    path = os.path.join(ROOT, "profiles", "op_adj_3d_192_16coils_pmc_traffic.json")
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as f:
+        d = json.load(f)
+    st = d.get("stages", {}).get(stage)
+    if st is None:
+        return None, None
+    return float(st["bytes"]), os.path.relpath(path, ROOT)
+
+
 def cpu_baseline(traj32, img, smaps, budget_note):
     """numpy oracle on one coil, full FFTs, 1/16 of the points; extrapolated."""
     sys.path.insert(0, ROOT)
@@ -260,6 +286,8 @@ def main():
     T = op.n_trans
     alg = stage_bytes(dom, op, T)
     achieved = alg / avg_s / 1e9
+    traffic, traffic_src = measured_traffic(dom)
+    pbytes = path_bytes(op, C)
     stage_summary = {s: {"ms_total": round(v[0], 3), "launches": v[1],
                          "ms_per_launch": round(v[0] / max(v[1], 1), 4)}
                      for s, v in timings.items() if v[1]}
@@ -293,6 +321,24 @@ def main():
           "runs": len(lat), "iterations": args.cg_iter, "graph": True}
     log(f"cg latency {cg}")
 
+    # the same CG with Toeplitz gradients (FFT-only iterations)
+    lat_t = []
+    rec_t = None
+    for r in range(args.cg_runs):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        rec_t = op.pinv_solver(y_obs, optim="cg", max_iter=args.cg_iter, graph=True, toeplitz=True)
+        torch.cuda.synchronize()
+        lat_t.append((time.perf_counter() - t) * 1e3)
+    if lat_t:
+        lt = np.array(lat_t)
+        cg["toeplitz"] = {"p50": float(np.percentile(lt, 50)), "p99": float(np.percentile(lt, 99)),
+                          "runs": len(lat_t),
+                          "first_ms": lat_t[0],
+                          "rel_l2_vs_nufft_cg": float(torch.linalg.vector_norm(rec_t - rec)
+                                                      / torch.linalg.vector_norm(rec))}
+        log(f"cg toeplitz {cg['toeplitz']}")
+
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline (bounded sample)")
@@ -322,8 +368,14 @@ def main():
         "stages": stage_summary,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+                     "frac": round(achieved / PEAK_HBM_GBS, 4),
+                     "traffic": None if traffic is None else int(traffic),
+                     "traffic_source": traffic_src,
                      "alg_bytes_per_launch": int(alg), "ms_per_launch": round(avg_s * 1e3, 4)},
+        "path_roofline": {"alg_bytes_per_op_adj": int(pbytes),
+                          "achieved": round(pbytes / (elapsed / args.steps) / 1e9, 1),
+                          "unit": "GB/s",
+                          "frac": round(pbytes / (elapsed / args.steps) / 1e9 / PEAK_HBM_GBS, 4)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

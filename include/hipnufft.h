@@ -203,6 +203,32 @@ int hipnufft_set_timing(hipnufft_plan* plan, int enable);
 int hipnufft_get_timing(hipnufft_plan* plan, double* ms, int64_t* launches,
                         int nstage);
 
+/* ---- Toeplitz (Gram) operator ---------------------------------------------
+ * G x = scale * A^H W A x for the type-2 transform A of the M points
+ * x_d[M] (radians, device) with weights W = diag(weights_d) (NULL: ones),
+ * applied as a circulant convolution on a 2N grid: pad, FFT, multiply by the
+ * kernel spectrum, inverse FFT, crop -- no spreading or interpolation per
+ * application.  The kernel is built once by one type-1 NUFFT of the weights
+ * onto the 2N grid (at tolerance eps) and one FFT.  2N must be 2,3,5-smooth
+ * on every axis.  With scale = 1/(prod(N) 2^ndim) and W the density,
+ * G equals adj_op(op(.)) of the normalised operator.  Replaces
+ * compute_toeplitz_kernel / apply_toeplitz_kernel and the gram_op family
+ * (operators/toeplitz.py:35-269, :337-468; base.py:316-405). */
+typedef struct hipnufft_gram_s hipnufft_gram;
+int hipnufft_gram_create(int ndim, const int64_t* nmodes, int64_t M,
+                         const float* x0_d, const float* x1_d,
+                         const float* x2_d, const float* weights_d,
+                         double eps, int iflag, float scale, int ntrans,
+                         void* stream, hipnufft_gram** out);
+/* SENSE Gram over B batches and C coils (ntrans bounds the workspace):
+ *   out_d[b] = sum_c conj(smaps_d[c]) * G(smaps_d[c] * img_d[b])
+ *   (out_d[b][c] = G(img_d[b][c]) if smaps_d == NULL). */
+int hipnufft_gram_apply(hipnufft_gram* gram, const void* img_d,
+                        const void* smaps_d, void* out_d, int B, int C);
+int hipnufft_gram_set_stream(hipnufft_gram* gram, void* stream);
+int hipnufft_gram_workspace_bytes(const hipnufft_gram* gram, size_t* bytes);
+int hipnufft_gram_destroy(hipnufft_gram* gram);
+
 /* Change the stream of subsequent calls. */
 int hipnufft_set_stream(hipnufft_plan* plan, void* stream);
 

@@ -167,6 +167,8 @@ int floor_pow2(int64_t x) {
   return p;
 }
 
+}  // namespace
+
 void free_plan(hipnufft_plan_s* p) {
   if (!p) return;
   for (auto* v : {&p->ev_used, &p->ev_free})
@@ -199,6 +201,8 @@ void free_plan(hipnufft_plan_s* p) {
   (void)hipFree(p->ksorted);
   delete p;
 }
+
+namespace {
 
 int need_pts(const hipnufft_plan_s* p) {
   if (!p) return set_error(HIPNUFFT_ERR_ARG, "null plan");
@@ -352,9 +356,68 @@ int dot_c64(const float2* a, const float2* b, int64_t rows, int64_t n, int conj,
   return 0;
 }
 
+int create_fft_plan(int ndim, const int64_t* nmodes, const int64_t* nf, int ntrans, hipStream_t s,
+                    hipnufft_plan_s** out) {
+  *out = nullptr;
+  hipnufft_plan_s* p = new (std::nothrow) hipnufft_plan_s();
+  if (!p) return set_error(HIPNUFFT_ERR_ALLOC, "host allocation failed");
+  p->ndim = ndim;
+  p->ntrans = ntrans;
+  p->stream = s;
+  p->sign2 = -1;
+  const int off = 3 - ndim;
+  int rc = 0;
+  for (int d = 0; d < ndim; ++d) {
+    if (nmodes[d] < 1 || nf[d] < nmodes[d] || (nf[d] & 1)) rc = set_error(HIPNUFFT_ERR_ARG, "invalid FFT grid");
+    p->N[off + d] = nmodes[d];
+    p->nf[off + d] = nf[d];
+  }
+  for (int d = 0; d < 3 && !rc; ++d) {
+    std::vector<float> dv(p->N[d], 1.0f);
+    if (hipMalloc((void**)&p->dec[d], sizeof(float) * dv.size()) != hipSuccess ||
+        hipMemcpy(p->dec[d], dv.data(), sizeof(float) * dv.size(), hipMemcpyHostToDevice) !=
+            hipSuccess)
+      rc = set_error(HIPNUFFT_ERR_ALLOC, "cannot allocate deconvolution tables");
+    p->bytes += sizeof(float) * dv.size();
+  }
+  for (int d = off; d < 3 && !rc; ++d) rc = make_axis(p->nf[d], &p->fft[d], &p->bytes);
+  while (!rc && p->fft[2].L > 1 && rows_adj_lds(p->fft[2], p->N[2]) > 65536) p->fft[2].L >>= 1;
+  if (!rc && rows_adj_lds(p->fft[2], p->N[2]) > 65536)
+    rc = set_error(HIPNUFFT_ERR_ARG, "grid too long along the last axis (max 2048)");
+  if (!rc) {
+    const size_t g = sizeof(float2) * (size_t)grid_size(p) * (size_t)ntrans;
+    if (hipMalloc((void**)&p->grid, g) != hipSuccess)
+      rc = set_error(HIPNUFFT_ERR_ALLOC, "cannot allocate the grid workspace");
+    p->bytes += g;
+  }
+  if (rc) {
+    free_plan(p);
+    return rc;
+  }
+  *out = p;
+  return 0;
+}
+
 }  // namespace hn
 
+// Toeplitz (Gram) operator: the FFT plan of the images in a 2N grid and the
+// spectrum of the circulant embedding of the point-spread kernel.
+struct hipnufft_gram_s {
+  hipnufft_plan_s* fft = nullptr;
+  float2* K = nullptr;
+  size_t bytes = 0;
+};
+
 using namespace hn;
+
+namespace {
+
+__global__ void k_cplx_weights(const float* w, int64_t M, float2* out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < M) out[i] = make_float2(w ? w[i] : 1.0f, 0.0f);
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -716,6 +779,118 @@ int hipnufft_plan_destroy(hipnufft_plan* p) {
   if (!p) return 0;
   (void)hipStreamSynchronize(p->stream);
   free_plan(p);
+  return 0;
+}
+
+int hipnufft_gram_create(int ndim, const int64_t* nmodes, int64_t M, const float* x0,
+                         const float* x1, const float* x2, const float* weights, double eps,
+                         int iflag, float scale, int ntrans, void* stream, hipnufft_gram** out) {
+  g_last_error.clear();
+  if (!out) return set_error(HIPNUFFT_ERR_ARG, "null output pointer");
+  *out = nullptr;
+  if (ndim < 1 || ndim > 3 || !nmodes) return set_error(HIPNUFFT_ERR_ARG, "ndim must be 1, 2 or 3");
+  if (ntrans < 1) return set_error(HIPNUFFT_ERR_ARG, "ntrans must be >= 1");
+  if (M < 0) return set_error(HIPNUFFT_ERR_ARG, "negative number of points");
+  hipStream_t s = (hipStream_t)stream;
+  int64_t n2[3], G2 = 1;
+  for (int d = 0; d < ndim; ++d) {
+    if (nmodes[d] < 1) return set_error(HIPNUFFT_ERR_ARG, "invalid mode count");
+    n2[d] = 2 * nmodes[d];
+    if (next_smooth_even(n2[d]) != n2[d])
+      return set_error(HIPNUFFT_ERR_ARG, "Toeplitz operator needs 2N to be 2,3,5-smooth on every axis");
+    G2 *= n2[d];
+  }
+  // 1. kernel kappa(l) = sum_m w_m exp(-iflag i omega_m . l), lags l in [-N, N):
+  //    one type-1 NUFFT of the weights onto the 2N image grid
+  hipnufft_plan_s* nu = nullptr;
+  int rc = hipnufft_plan_create(2, ndim, n2, iflag, 1, eps, nullptr, stream, &nu);
+  if (rc) return rc;
+  float2 *w = nullptr, *kappa = nullptr;
+  rc = hipnufft_setpts(nu, M, x0, x1, x2);
+  if (!rc && (hipMalloc((void**)&w, sizeof(float2) * std::max<int64_t>(M, 1)) != hipSuccess ||
+              hipMalloc((void**)&kappa, sizeof(float2) * G2) != hipSuccess))
+    rc = set_error(HIPNUFFT_ERR_ALLOC, "cannot allocate the Toeplitz kernel");
+  if (!rc && M > 0) {
+    hipLaunchKernelGGL(k_cplx_weights, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, s, weights,
+                       M, w);
+    if (hipGetLastError() != hipSuccess) rc = set_error(HIPNUFFT_ERR_HIP, "kernel launch failed");
+  }
+  if (!rc) rc = hipnufft_sense_adj(nu, w, nullptr, nullptr, kappa, 1, 1, 1.0f);
+  // 2. spectrum of the circulant: the centred placement of the 2N FFT plan
+  //    puts lag l at position l mod 2N, so its forward chain gives
+  //    K = FFT(circulant) directly; the inverse's 1/prod(2N) and the
+  //    operator scale are folded in
+  hipnufft_plan_s* kp = nullptr;
+  if (!rc) rc = create_fft_plan(ndim, n2, n2, 1, s, &kp);
+  if (!rc) rc = fwd_to_grid(kp, kappa, nullptr, 1, 0, 1, (float)((double)scale / (double)G2), -1);
+  hipnufft_gram_s* g = nullptr;
+  if (!rc) {
+    g = new (std::nothrow) hipnufft_gram_s();
+    if (!g) rc = set_error(HIPNUFFT_ERR_ALLOC, "host allocation failed");
+  }
+  if (!rc) {
+    g->K = kp->grid;  // take the spectrum
+    kp->grid = nullptr;
+    g->bytes = sizeof(float2) * G2;
+    rc = create_fft_plan(ndim, nmodes, n2, ntrans, s, &g->fft);
+  }
+  (void)hipStreamSynchronize(s);
+  (void)hipFree(w);
+  (void)hipFree(kappa);
+  if (kp) free_plan(kp);
+  if (nu) free_plan(nu);
+  if (rc) {
+    if (g) {
+      (void)hipFree(g->K);
+      if (g->fft) free_plan(g->fft);
+      delete g;
+    }
+    return rc;
+  }
+  g->bytes += g->fft->bytes;
+  *out = g;
+  return 0;
+}
+
+int hipnufft_gram_apply(hipnufft_gram* g, const void* img, const void* smaps, void* out, int B,
+                        int C) {
+  if (!g || !g->fft) return set_error(HIPNUFFT_ERR_ARG, "null Toeplitz operator");
+  if (B < 1 || C < 1) return set_error(HIPNUFFT_ERR_ARG, "B and C must be >= 1");
+  if (!img || !out) return set_error(HIPNUFFT_ERR_ARG, "null data pointer");
+  hipnufft_plan_s* p = g->fft;
+  const float2* x = (const float2*)img;
+  const float2* sm = (const float2*)smaps;
+  std::vector<Chunk> chunks;
+  int rc = make_chunks(p, B, C, sm != nullptr, chunks);
+  if (rc) return rc;
+  for (const Chunk& c : chunks) {
+    if ((rc = fwd_to_grid(p, x, sm, C, c.t0, c.T, 1.0f, -1))) return rc;
+    if ((rc = mul_spectrum(p, g->K, c.T))) return rc;
+    if ((rc = grid_to_img(p, (float2*)out, sm, C, c.t0, c.T, 1.0f, 1))) return rc;
+  }
+  return 0;
+}
+
+int hipnufft_gram_set_stream(hipnufft_gram* g, void* stream) {
+  if (!g || !g->fft) return set_error(HIPNUFFT_ERR_ARG, "null Toeplitz operator");
+  g->fft->stream = (hipStream_t)stream;
+  return 0;
+}
+
+int hipnufft_gram_workspace_bytes(const hipnufft_gram* g, size_t* bytes) {
+  if (!g || !bytes) return set_error(HIPNUFFT_ERR_ARG, "null pointer");
+  *bytes = g->bytes;
+  return 0;
+}
+
+int hipnufft_gram_destroy(hipnufft_gram* g) {
+  if (!g) return 0;
+  if (g->fft) {
+    (void)hipStreamSynchronize(g->fft->stream);
+    free_plan(g->fft);
+  }
+  (void)hipFree(g->K);
+  delete g;
   return 0;
 }
 

@@ -508,6 +508,27 @@ RowsArgs rows_args(const hipnufft_plan_s* p) {
 
 int64_t grid_size(const hipnufft_plan_s* p) { return p->nf[0] * p->nf[1] * p->nf[2]; }
 
+namespace {
+
+// grid[t][i] *= K[i] for the T grids of a chunk (the Toeplitz spectrum)
+__global__ __launch_bounds__(kThreads) void k_mul_spectrum(float2* grid, const float2* K, int64_t G) {
+  const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (i >= G) return;
+  float2* g = grid + (int64_t)blockIdx.y * G;
+  g[i] = cmul(g[i], K[i]);
+}
+
+}  // namespace
+
+int mul_spectrum(const hipnufft_plan_s* p, const float2* K, int T) {
+  const int64_t G = grid_size(p);
+  if (T <= 0 || G <= 0) return 0;
+  const dim3 grd((unsigned)((G + kThreads - 1) / kThreads), (unsigned)T);
+  hipLaunchKernelGGL(k_mul_spectrum, grd, dim3(kThreads), 0, p->stream, p->grid, K, G);
+  HN_LAUNCH_CHECK();
+  return 0;
+}
+
 int fwd_to_grid(const hipnufft_plan_s* p, const float2* img, const float2* smaps, int C, int t0,
                 int T, float scale, int sign, int conj_smaps, int smaps_all) {
   if (T <= 0) return 0;

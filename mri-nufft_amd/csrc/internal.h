@@ -243,6 +243,15 @@ int grid_to_img(const hipnufft_plan_s* p, float2* img, const float2* smaps,
                 int C, int t0, int T, float scale, int sign, int smaps_all = 0);
 
 // rows independent FP64 dots of a[rows][n] and b[rows][n] -> out[rows][2]
+// grid[t] *= K (elementwise complex) for the T grids of the plan workspace
+int mul_spectrum(const hipnufft_plan_s* p, const float2* K, int T);
+
+// FFT-only plan (no points): images of nmodes embedded, centred, in grids of
+// nf >= nmodes per axis, unit deconvolution (the Toeplitz operator's FFTs)
+int create_fft_plan(int ndim, const int64_t* nmodes, const int64_t* nf, int ntrans,
+                    hipStream_t s, hipnufft_plan_s** out);
+void free_plan(hipnufft_plan_s* p);
+
 int dot_c64(const float2* a, const float2* b, int64_t rows, int64_t n, int conj,
             double* out, double* ws, hipStream_t s);
 

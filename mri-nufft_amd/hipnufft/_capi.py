@@ -75,6 +75,12 @@ _SIGNATURES = {
     "hipnufft_set_timing": [_P, _I],
     "hipnufft_get_timing": [_P, _P, _P, _I],
     "hipnufft_plan_destroy": [_P],
+    "hipnufft_gram_create": [_I, ctypes.POINTER(_I64), _I64, _P, _P, _P, _P, ctypes.c_double, _I,
+                             _F, _I, _P, ctypes.POINTER(_P)],
+    "hipnufft_gram_apply": [_P, _P, _P, _P, _I, _I],
+    "hipnufft_gram_set_stream": [_P, _P],
+    "hipnufft_gram_workspace_bytes": [_P, ctypes.POINTER(ctypes.c_size_t)],
+    "hipnufft_gram_destroy": [_P],
 }
 
 EXPORTED = sorted(list(_SIGNATURES) + ["hipnufft_last_error"])
@@ -276,6 +282,53 @@ class RawPlan:
         h, self._h = getattr(self, "_h", None), None
         if h and _lib is not None:
             _lib.hipnufft_plan_destroy(h)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class RawGram:
+    """Owns one ``hipnufft_gram`` (Toeplitz form of ``scale * A^H W A``)."""
+
+    def __init__(self, shape, coords, weights=None, eps=1e-6, isign=-1, scale=1.0, n_trans=1,
+                 stream=None):
+        L = lib()
+        self.shape = tuple(int(s) for s in shape)
+        self.ndim = len(self.shape)
+        self.stream = current_stream() if stream is None else stream
+        c = coords.to(torch.float32)
+        cols = [c[:, d].contiguous() for d in range(self.ndim)]
+        args = [ptr(x) for x in cols] + [None] * (3 - self.ndim)
+        w = None if weights is None else weights.to(torch.float32).contiguous()
+        nm = (_I64 * self.ndim)(*self.shape)
+        h = _P()
+        check(L.hipnufft_gram_create(self.ndim, nm, c.shape[0], *args, ptr(w), float(eps),
+                                     int(isign), float(scale), int(n_trans), self.stream,
+                                     ctypes.byref(h)), "gram_create")
+        self._h = h
+
+    def _sync_stream(self):
+        s = current_stream()
+        if s != self.stream:
+            self.stream = s
+            check(lib().hipnufft_gram_set_stream(self._h, s))
+
+    def apply(self, img, smaps, out, B, C):
+        self._sync_stream()
+        check(lib().hipnufft_gram_apply(self._h, ptr(img), ptr(smaps), ptr(out), B, C), "gram")
+
+    def workspace_bytes(self):
+        b = ctypes.c_size_t()
+        check(lib().hipnufft_gram_workspace_bytes(self._h, ctypes.byref(b)))
+        return b.value
+
+    def close(self):
+        h, self._h = getattr(self, "_h", None), None
+        if h and _lib is not None:
+            _lib.hipnufft_gram_destroy(h)
 
     def __del__(self):
         try:

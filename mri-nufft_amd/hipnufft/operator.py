@@ -185,6 +185,7 @@ class MRIHipNUFFT(FourierOperatorBase, _ToggleGradPlanMixin):
 
     @density.setter
     def density(self, new_density):
+        self._toeplitz_kernel = None  # the Toeplitz kernel holds the weights
         if new_density is None:
             self._density = None
             return
@@ -211,6 +212,7 @@ class MRIHipNUFFT(FourierOperatorBase, _ToggleGradPlanMixin):
             else:
                 self._samples = new_samples
             self.raw_op.set_pts(self._samples)
+            self._toeplitz_kernel = None
             if self._density_method is not None:
                 self.compute_density(self._density_method)
 
@@ -284,6 +286,35 @@ class MRIHipNUFFT(FourierOperatorBase, _ToggleGradPlanMixin):
         self.raw_op.plan.sense_adj(coeffs.resolve_conj().resolve_neg().contiguous(), None, None,
                                    image, T, 1, 1.0)
         return image
+
+    # ---------------------------------------------------------------- Gram
+    def compute_toeplitz_kernel(self):
+        """Build the Toeplitz form of ``adj_op(op(.))`` (density included):
+        one type-1 NUFFT of the weights onto a 2N grid and one FFT
+        (``base.py:400-405``, ``toeplitz.py:35-269``)."""
+        with torch.cuda.device(self.device):
+            self._toeplitz_kernel = _capi.RawGram(
+                self.shape, self._samples, self._density, eps=self._eps, isign=-1,
+                scale=self.inv_norm_factor ** 2, n_trans=self.n_trans)
+        return self._toeplitz_kernel
+
+    def gram_op(self, data, toeplitz=True):
+        """``A^H D A x`` -- by the Toeplitz kernel (no spreading or
+        interpolation) or, with ``toeplitz=False``, as ``adj_op(op(x))``
+        (``base.py:316-342``, ``toeplitz.py:376-468``)."""
+        self.check_shape(image=data)
+        if not toeplitz:
+            return self.adj_op(self.op(data))
+        kind = kind_of(data)
+        B, C = self.n_batchs, self.n_coils
+        ic = 1 if self.uses_sense else C
+        with torch.cuda.device(self.device):
+            if getattr(self, "_toeplitz_kernel", None) is None:
+                self.compute_toeplitz_kernel()
+            x = self._img_in(data, ic)
+            out = torch.empty((B, ic, *self.shape), dtype=torch.complex64, device=self.device)
+            self._toeplitz_kernel.apply(x, self._smaps, out, B, C)
+        return from_device(self._safe_squeeze(out), kind)
 
     # --------------------------------------------------------------- extras
     def get_lipschitz_cst(self, max_iter=10, seed=None, **kwargs):

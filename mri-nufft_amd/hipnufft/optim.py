@@ -60,8 +60,14 @@ def _with_damp(grad, image, damp, x0, shape):
 
 @register_optim
 def cg(operator, kspace_data, damp=0.0, x0=None, x_init=None, max_iter=10, tol=1e-4,
-       progressbar=False, callback=None, graph=False):
-    """Non-linear CG of ``optim.py:801-902`` on the device (FP64 inner products)."""
+       progressbar=False, callback=None, graph=False, toeplitz=False):
+    """Non-linear CG of ``optim.py:801-902`` on the device (FP64 inner products).
+
+    ``toeplitz=True`` evaluates each gradient ``A^H D (A x - y)`` as
+    ``G x - A^H D y`` with the operator's Toeplitz Gram form ``G`` (built
+    once, ``gram_op``) and ``A^H D y`` computed once: the iterations then run
+    FFTs only, no spreading or interpolation.  Same recurrences otherwise.
+    """
     kind = kind_of(kspace_data)
     dev = _dev(operator)
     y = to_device(kspace_data, torch.complex64, dev)
@@ -80,19 +86,27 @@ def cg(operator, kspace_data, damp=0.0, x0=None, x_init=None, max_iter=10, tol=1
             operator.density = None
         image = (torch.zeros(shape, dtype=torch.complex64, device=dev) if x_init is None
                  else to_device(x_init, torch.complex64, dev).reshape(shape).clone())
+        if toeplitz:
+            rhs = to_device(operator.adj_op(y), torch.complex64, dev).reshape(shape)
+
+            def dc(img):
+                return to_device(operator.gram_op(img), torch.complex64, dev).reshape(shape) - rhs
+        else:
+            def dc(img):
+                return operator.data_consistency(img, y).reshape(shape)
         velocity = torch.zeros_like(image)
-        grad = operator.data_consistency(image, y).reshape(shape)
+        grad = dc(image)
         grad = _with_damp(grad, image, damp, x0, shape)
         velocity = tol * velocity + grad / lipschitz_cst
         image = image - velocity
         if graph and callback is None:
-            image = _cg_graph(operator, y, image, grad, velocity, lipschitz_cst, damp, x0,
+            image = _cg_graph(dc, image, grad, velocity, lipschitz_cst, damp, x0,
                               max_iter, tol, shape)
             callbacks_results = []
         else:
             callbacks_results = []
             for _ in range(max_iter):
-                grad_new = operator.data_consistency(image, y).reshape(shape)
+                grad_new = dc(image)
                 grad_new = _with_damp(grad_new, image, damp, x0, shape)
                 nrm2 = _dot(grad_new, grad_new, True)
                 num = _dot(grad_new, grad_new - grad, False)
@@ -120,7 +134,7 @@ def cg(operator, kspace_data, damp=0.0, x0=None, x_init=None, max_iter=10, tol=1
     return image
 
 
-def _cg_graph(operator, y, image, grad, velocity, L, damp, x0, max_iter, tol, shape):
+def _cg_graph(dc, image, grad, velocity, L, damp, x0, max_iter, tol, shape):
     """CG iterations with a device stop flag, one iteration captured in a HIP graph."""
     dev = image.device
     st = dict(image=image.contiguous(), grad=grad.contiguous(), vel=velocity.contiguous(),
@@ -134,8 +148,7 @@ def _cg_graph(operator, y, image, grad, velocity, L, damp, x0, max_iter, tol, sh
     tol2 = tol * tol
 
     def step():
-        gn = operator.data_consistency(st["image"], y).reshape(shape)
-        gn = _with_damp(gn, st["image"], damp, x0, shape)
+        gn = _with_damp(dc(st["image"]), st["image"], damp, x0, shape)
         ws = bufs["ws"]
         _capi.dot_c64(gn.reshape(-1), gn.reshape(-1), True, out=bufs["nrm"], ws=ws)
         diff = gn - st["grad"]

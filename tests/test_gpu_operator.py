@@ -348,3 +348,50 @@ def test_autograd_traj_central_fd():
             vals.append(float(torch.sum(torch.abs(op2.op(x) - tgt) ** 2)))
         fd = (vals[0] - vals[1]) / (2 * h)
         assert abs(fd - float(gt[m, d])) <= 2e-2 * max(1.0, abs(fd))
+
+
+@pytest.mark.parametrize("dim,sense,density", [(2, True, False), (2, False, True),
+                                               (3, True, True), (2, True, True)])
+def test_gram_toeplitz_matches_adj_op(dim, sense, density):
+    """gram_op (Toeplitz kernel) == adj_op(op(x)) (reference test_batch.py:275-293:
+    rtol 2e-4, atol 1e-4)."""
+    from hipnufft import MRIHipNUFFT
+
+    shape = (48, 40) if dim == 2 else (16, 20, 12)
+    traj = truncnorm_traj(3000, dim, 90)
+    C = 3
+    smaps = gauss_smaps(C, shape, seed=91) if sense else None
+    dens = np.random.default_rng(92).uniform(0.2, 2.0, traj.shape[0]).astype(np.float32) \
+        if density else False
+    op = MRIHipNUFFT(traj, shape, n_coils=C, n_batchs=2, smaps=smaps, density=dens)
+    ic = 1 if sense else C
+    x = torch.from_numpy(rand_cplx((2, ic, *shape), seed=93)).cuda()
+    ref = op.adj_op(op.op(x)).cpu().numpy()
+    g = op.gram_op(x).cpu().numpy()
+    assert g.shape == ref.shape
+    np.testing.assert_allclose(g, ref, rtol=2e-4, atol=1e-4 * np.abs(ref).max())
+    assert rel_l2(g, ref) < 2e-5
+    # the direct path and a recomputed kernel after a density change
+    np.testing.assert_allclose(op.gram_op(x, toeplitz=False).cpu().numpy(), ref, rtol=1e-6, atol=1e-6)
+    op.density = None
+    ref2 = op.adj_op(op.op(x)).cpu().numpy()
+    assert rel_l2(op.gram_op(x).cpu().numpy(), ref2) < 2e-5
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_cg_toeplitz_matches_nufft_cg(graph):
+    """CG with Toeplitz gradients equals CG with NUFFT data consistency
+    (same L, same recurrences) to the 1e-4 reconstruction bar."""
+    from hipnufft import MRIHipNUFFT, cg
+
+    shape = (40, 48)
+    traj = radial2d(40, 160)
+    smaps = gauss_smaps(4, shape, seed=95)
+    op = MRIHipNUFFT(traj, shape, n_coils=4, smaps=smaps)
+    x = torch.from_numpy(rand_cplx(shape, seed=96)).cuda()
+    y = op.op(x)
+    L = op.get_lipschitz_cst(seed=0)
+    op.get_lipschitz_cst = lambda **_: L
+    a = cg(op, y, max_iter=12, graph=graph).cpu().numpy()
+    b = cg(op, y, max_iter=12, graph=graph, toeplitz=True).cpu().numpy()
+    assert rel_l2(b, a) < 1e-4
