@@ -144,6 +144,8 @@ int make_axis(int64_t n, FFTAxis* ax, size_t* bytes) {
   while (m % 3 == 0 && ax->nrad < kMaxRadices) { ax->rad[ax->nrad++] = 3; m /= 3; }
   while (m % 5 == 0 && ax->nrad < kMaxRadices) { ax->rad[ax->nrad++] = 5; m /= 5; }
   if (m != 1) return set_error(HIPNUFFT_ERR_ARG, "fine grid size is not 2,3,5-smooth");
+  for (int i = 0, ns = 1; i < ax->nrad; ns *= ax->rad[i], ++i)
+    ax->ns_magic[i] = ns > 1 ? (uint32_t)((((uint64_t)1 << 32) + ns - 1) / ns) : 0u;
   std::vector<float2> tw(n);
   for (int64_t t = 0; t < n; ++t) {
     const double a = -2.0 * M_PI * (double)t / (double)n;
@@ -863,11 +865,8 @@ int hipnufft_gram_apply(hipnufft_gram* g, const void* img, const void* smaps, vo
   std::vector<Chunk> chunks;
   int rc = make_chunks(p, B, C, sm != nullptr, chunks);
   if (rc) return rc;
-  for (const Chunk& c : chunks) {
-    if ((rc = fwd_to_grid(p, x, sm, C, c.t0, c.T, 1.0f, -1))) return rc;
-    if ((rc = mul_spectrum(p, g->K, c.T))) return rc;
-    if ((rc = grid_to_img(p, (float2*)out, sm, C, c.t0, c.T, 1.0f, 1))) return rc;
-  }
+  for (const Chunk& c : chunks)
+    if ((rc = gram_chain(p, x, sm, (float2*)out, g->K, C, c.t0, c.T))) return rc;
   return 0;
 }
 

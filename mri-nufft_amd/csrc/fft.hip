@@ -122,7 +122,7 @@ __device__ __forceinline__ void dftR(float2* v) {
 // registers, then (after a barrier) writes the results: a single LDS buffer.
 template <int R, int S>
 __device__ __forceinline__ void stockham_pass(float2* buf, int n, int lgL, int LP, int Ns,
-                                              const float2* tw) {
+                                              uint32_t ns_magic, const float2* tw) {
   constexpr int MAXB = (kEPT + R - 1) / R;
   const int L = 1 << lgL;
   const int nr = n / R;
@@ -144,13 +144,16 @@ __device__ __forceinline__ void stockham_pass(float2* buf, int n, int lgL, int L
     const int b = threadIdx.x + i * kThreads;
     if (b < nbf) {
       const int l = b & (L - 1), j = b >> lgL;
-      const int q = j / Ns;
+      // q = j / Ns by multiply-high (exact for j < 2^16); Ns = 1 in the first pass
+      const int q = Ns == 1 ? j : (int)__umulhi((uint32_t)j, ns_magic);
       const int k = j - q * Ns;
+      if (Ns > 1) {  // the first pass has unit twiddles
 #pragma unroll
-      for (int r = 1; r < R; ++r) {
-        float2 w = tw[r * k * tstep];
-        if (S > 0) w.y = -w.y;
-        v[i][r] = cmul(v[i][r], w);
+        for (int r = 1; r < R; ++r) {
+          float2 w = tw[r * k * tstep];
+          if (S > 0) w.y = -w.y;
+          v[i][r] = cmul(v[i][r], w);
+        }
       }
       dftR<R, S>(v[i]);
       const int base = q * Ns * R + k;
@@ -168,12 +171,13 @@ __device__ void fft_lines(float2* buf, const float2* twl, const FFTAxis& ax, int
   int Ns = 1;
   for (int p = 0; p < ax.nrad; ++p) {
     const int R = ax.rad[p];
+    const uint32_t mg = ax.ns_magic[p];
     switch (R) {
-      case 2: stockham_pass<2, S>(buf, ax.n, lgL, LP, Ns, twl); break;
-      case 3: stockham_pass<3, S>(buf, ax.n, lgL, LP, Ns, twl); break;
-      case 4: stockham_pass<4, S>(buf, ax.n, lgL, LP, Ns, twl); break;
-      case 5: stockham_pass<5, S>(buf, ax.n, lgL, LP, Ns, twl); break;
-      default: stockham_pass<8, S>(buf, ax.n, lgL, LP, Ns, twl); break;
+      case 2: stockham_pass<2, S>(buf, ax.n, lgL, LP, Ns, mg, twl); break;
+      case 3: stockham_pass<3, S>(buf, ax.n, lgL, LP, Ns, mg, twl); break;
+      case 4: stockham_pass<4, S>(buf, ax.n, lgL, LP, Ns, mg, twl); break;
+      case 5: stockham_pass<5, S>(buf, ax.n, lgL, LP, Ns, mg, twl); break;
+      default: stockham_pass<8, S>(buf, ax.n, lgL, LP, Ns, mg, twl); break;
     }
     Ns *= R;
   }
@@ -220,8 +224,9 @@ __global__ __launch_bounds__(kThreads) void k_rows_fwd(RowsArgs a) {
   const int L = 1 << a.lgL, LP = L + 1;
   const float2* twl = stage_twiddles(buf, a.ax, LP);
   const int rows = a.M0 * a.M1;
-  const int r0 = blockIdx.x * L;
-  const int t = blockIdx.y;
+  const BlockIdx3 bi = remapped_block();
+  const int r0 = bi.x * L;
+  const int t = bi.y;
   const int tt = a.t0 + t;
   const int64_t N = (int64_t)rows * a.M2;
   const float2* src = a.img + (int64_t)(a.smaps ? tt / a.C : tt) * N;
@@ -286,17 +291,18 @@ __global__ __launch_bounds__(kThreads) void k_rows_adj(RowsArgs a) {
   const int L = 1 << a.lgL, LP = L + 1;
   const float2* twl = stage_twiddles(buf, a.ax, LP);
   const int rows = a.M0 * a.M1;
-  const int r0 = blockIdx.x * L;
+  const BlockIdx3 bi = remapped_block();
+  const int r0 = bi.x * L;
   const int n = a.nf2;
   const int64_t N = (int64_t)rows * a.M2;
   int tb, te, out_img;
   if (a.smaps) {  // one output image per batch present in the chunk
-    const int b = a.t0 / a.C + blockIdx.y;
+    const int b = a.t0 / a.C + bi.y;
     out_img = b;
     tb = max(a.t0, b * a.C);
     te = min(a.t0 + a.T, (b + 1) * a.C);
   } else {
-    out_img = a.t0 + blockIdx.y;
+    out_img = a.t0 + bi.y;
     tb = out_img;
     te = out_img + 1;
   }
@@ -383,16 +389,21 @@ struct StridedArgs {
   int in_M, out_M;  // 0: full line, > 0: kept positions only
   FFTAxis ax;
   int lgL;
+  const float2* K;  // CONV: spectrum (one grid) applied between the FFTs
 };
 
-template <int S>
+// In-place FFT of L adjacent lines along a strided axis.  CONV: forward FFT,
+// multiply by the spectrum K, inverse FFT, all in LDS (the middle of the
+// Toeplitz convolution, one grid round trip instead of three).
+template <int S, bool CONV>
 __global__ __launch_bounds__(kThreads) void k_fft_strided(StridedArgs a) {
   extern __shared__ float2 buf[];
   const int L = 1 << a.lgL, LP = L + 1;
   const float2* twl = stage_twiddles(buf, a.ax, LP);
-  const int x0 = blockIdx.x * L;
-  const int oi = blockIdx.y;
-  const int t = blockIdx.z;
+  const BlockIdx3 bi = remapped_block();
+  const int x0 = bi.x * L;
+  const int oi = bi.y;
+  const int t = bi.z;
   const int opos = a.outer_M ? kept_pos(oi, a.outer_M, a.outer_nf) : oi;
   float2* base = a.data + (int64_t)t * a.tstride + (int64_t)opos * a.ostride + x0;
   const int n = a.n;
@@ -428,6 +439,26 @@ __global__ __launch_bounds__(kThreads) void k_fft_strided(StridedArgs a) {
   }
   __syncthreads();
   fft_lines<S>(buf, twl, a.ax, a.lgL, LP);
+  if constexpr (CONV) {
+    const float2* kb = a.K + (int64_t)opos * a.ostride + x0;
+#pragma unroll
+    for (int i = 0; i < kEPT; ++i) {
+      const int idx = threadIdx.x + i * kThreads;
+      const int l = idx & (L - 1), e = idx >> a.lgL;
+      v[i] = make_float2(0.f, 0.f);
+      if (idx < n * L && x0 + l < a.nx) v[i] = kb[(int64_t)e * a.lstride + l];
+    }
+#pragma unroll
+    for (int i = 0; i < kEPT; ++i) {
+      const int idx = threadIdx.x + i * kThreads;
+      if (idx < n * L) {
+        const int l = idx & (L - 1), e = idx >> a.lgL;
+        buf[e * LP + l] = cmul(buf[e * LP + l], v[i]);
+      }
+    }
+    __syncthreads();
+    fft_lines<-S>(buf, twl, a.ax, a.lgL, LP);
+  }
   const int nout = a.out_M ? a.out_M : n;
   for (int idx = threadIdx.x; idx < nout * L; idx += kThreads) {
     const int l = idx & (L - 1), e = idx >> a.lgL;
@@ -454,7 +485,7 @@ size_t rows_adj_lds(const FFTAxis& ax, int64_t m2) {
 namespace {
 
 int strided(const hipnufft_plan_s* p, int axis, int T, int in_M, int out_M, int outer_M,
-            int sign, int stage) {
+            int sign, int stage, const float2* K = nullptr) {
   const FFTAxis& ax = p->fft[axis];
   StridedArgs a{};
   a.data = p->grid;
@@ -476,12 +507,19 @@ int strided(const hipnufft_plan_s* p, int axis, int T, int in_M, int out_M, int 
   }
   a.outer_M = outer_M;
   a.nouter = outer_M ? outer_M : a.outer_nf;
+  a.K = K;
   const dim3 grd((unsigned)((a.nx + ax.L - 1) / ax.L), (unsigned)a.nouter, (unsigned)T);
   StageScope timer(p, stage);
-  if (sign < 0)
-    hipLaunchKernelGGL(k_fft_strided<-1>, grd, dim3(kThreads), fft_lds(ax), p->stream, a);
-  else
-    hipLaunchKernelGGL(k_fft_strided<1>, grd, dim3(kThreads), fft_lds(ax), p->stream, a);
+  if (K) {
+    if (sign < 0)
+      hipLaunchKernelGGL((k_fft_strided<-1, true>), grd, dim3(kThreads), fft_lds(ax), p->stream, a);
+    else
+      hipLaunchKernelGGL((k_fft_strided<1, true>), grd, dim3(kThreads), fft_lds(ax), p->stream, a);
+  } else if (sign < 0) {
+    hipLaunchKernelGGL((k_fft_strided<-1, false>), grd, dim3(kThreads), fft_lds(ax), p->stream, a);
+  } else {
+    hipLaunchKernelGGL((k_fft_strided<1, false>), grd, dim3(kThreads), fft_lds(ax), p->stream, a);
+  }
   HN_LAUNCH_CHECK();
   return 0;
 }
@@ -529,9 +567,10 @@ int mul_spectrum(const hipnufft_plan_s* p, const float2* K, int T) {
   return 0;
 }
 
-int fwd_to_grid(const hipnufft_plan_s* p, const float2* img, const float2* smaps, int C, int t0,
-                int T, float scale, int sign, int conj_smaps, int smaps_all) {
-  if (T <= 0) return 0;
+namespace {
+
+int rows_fwd(const hipnufft_plan_s* p, const float2* img, const float2* smaps, int C, int t0, int T,
+             float scale, int sign, int conj_smaps, int smaps_all) {
   RowsArgs a = rows_args(p);
   a.conj_smaps = conj_smaps;
   a.smaps_all = smaps_all;
@@ -553,24 +592,11 @@ int fwd_to_grid(const hipnufft_plan_s* p, const float2* img, const float2* smaps
       hipLaunchKernelGGL(k_rows_fwd<1>, grd, dim3(kThreads), lds, p->stream, a);
     HN_LAUNCH_CHECK();
   }
-  int rc;
-  if (p->ndim >= 2 &&
-      (rc = strided(p, 1, T, (int)p->N[1], 0, (int)p->N[0], sign, kStageColsFwd)))
-    return rc;
-  if (p->ndim == 3 && (rc = strided(p, 0, T, (int)p->N[0], 0, 0, sign, kStageSlabsFwd)))
-    return rc;
   return 0;
 }
 
-int grid_to_img(const hipnufft_plan_s* p, float2* img, const float2* smaps, int C, int t0, int T,
-                float scale, int sign, int smaps_all) {
-  if (T <= 0) return 0;
-  int rc;
-  if (p->ndim == 3 && (rc = strided(p, 0, T, 0, (int)p->N[0], 0, sign, kStageSlabsAdj)))
-    return rc;
-  if (p->ndim >= 2 &&
-      (rc = strided(p, 1, T, 0, (int)p->N[1], (int)p->N[0], sign, kStageColsAdj)))
-    return rc;
+int rows_adj(const hipnufft_plan_s* p, float2* img, const float2* smaps, int C, int t0, int T,
+             float scale, int sign, int smaps_all) {
   RowsArgs a = rows_args(p);
   a.img_out = img;
   a.smaps = smaps;
@@ -591,6 +617,55 @@ int grid_to_img(const hipnufft_plan_s* p, float2* img, const float2* smaps, int 
     hipLaunchKernelGGL(k_rows_adj<1>, grd, dim3(kThreads), lds, p->stream, a);
   HN_LAUNCH_CHECK();
   return 0;
+}
+
+}  // namespace
+
+int fwd_to_grid(const hipnufft_plan_s* p, const float2* img, const float2* smaps, int C, int t0,
+                int T, float scale, int sign, int conj_smaps, int smaps_all) {
+  if (T <= 0) return 0;
+  int rc = rows_fwd(p, img, smaps, C, t0, T, scale, sign, conj_smaps, smaps_all);
+  if (rc) return rc;
+  if (p->ndim >= 2 &&
+      (rc = strided(p, 1, T, (int)p->N[1], 0, (int)p->N[0], sign, kStageColsFwd)))
+    return rc;
+  if (p->ndim == 3 && (rc = strided(p, 0, T, (int)p->N[0], 0, 0, sign, kStageSlabsFwd)))
+    return rc;
+  return 0;
+}
+
+int grid_to_img(const hipnufft_plan_s* p, float2* img, const float2* smaps, int C, int t0, int T,
+                float scale, int sign, int smaps_all) {
+  if (T <= 0) return 0;
+  int rc;
+  if (p->ndim == 3 && (rc = strided(p, 0, T, 0, (int)p->N[0], 0, sign, kStageSlabsAdj)))
+    return rc;
+  if (p->ndim >= 2 &&
+      (rc = strided(p, 1, T, 0, (int)p->N[1], (int)p->N[0], sign, kStageColsAdj)))
+    return rc;
+  return rows_adj(p, img, smaps, C, t0, T, scale, sign, smaps_all);
+}
+
+int gram_chain(const hipnufft_plan_s* p, const float2* img, const float2* smaps, float2* out,
+               const float2* K, int C, int t0, int T) {
+  if (T <= 0) return 0;
+  int rc;
+  if (p->ndim == 1) {  // the only FFT axis is the contiguous one: unfused
+    if ((rc = fwd_to_grid(p, img, smaps, C, t0, T, 1.0f, -1))) return rc;
+    if ((rc = mul_spectrum(p, K, T))) return rc;
+    return grid_to_img(p, out, smaps, C, t0, T, 1.0f, 1);
+  }
+  if ((rc = rows_fwd(p, img, smaps, C, t0, T, 1.0f, -1, 0, 0))) return rc;
+  if (p->ndim == 3) {
+    if ((rc = strided(p, 1, T, (int)p->N[1], 0, (int)p->N[0], -1, kStageColsFwd))) return rc;
+    // axis 0: forward FFT, spectrum, inverse FFT in one pass
+    if ((rc = strided(p, 0, T, (int)p->N[0], (int)p->N[0], 0, -1, kStageSlabsFwd, K))) return rc;
+    if ((rc = strided(p, 1, T, 0, (int)p->N[1], (int)p->N[0], 1, kStageColsAdj))) return rc;
+  } else {
+    if ((rc = strided(p, 1, T, (int)p->N[1], (int)p->N[1], (int)p->N[0], -1, kStageColsFwd, K)))
+      return rc;
+  }
+  return rows_adj(p, out, smaps, C, t0, T, 1.0f, 1, 0);
 }
 
 }  // namespace hn

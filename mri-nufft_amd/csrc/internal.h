@@ -40,6 +40,7 @@ struct FFTAxis {
   int L = 1;          // lines per workgroup (power of two)
   int nrad = 0;
   int rad[kMaxRadices] = {0};
+  uint32_t ns_magic[kMaxRadices] = {0};  // ceil(2^32 / Ns) of each pass (Ns > 1)
   float2* tw = nullptr;  // tw[t] = exp(-2 pi i t / n), t < n
 };
 
@@ -194,6 +195,34 @@ size_t rows_adj_lds(const FFTAxis& ax, int64_t m2);
 // lists of the plan.  Synchronises the plan stream.
 int build_points(hipnufft_plan_s* p, int nsets, const int64_t* Ms, const float* const* xd);
 
+// Workgroups are dispatched round-robin over the 8 XCDs of an MI355X (each
+// with its own L2), so consecutive block ids land on different L2s.  This
+// bijection on [0, n) gives XCD x the contiguous logical range
+// [x*n/8, (x+1)*n/8): neighbouring tiles / column blocks, which share halo
+// cells or cache lines, then run on the same XCD.
+constexpr int kNumXCD = 8;
+__device__ __forceinline__ int64_t xcd_remap(int64_t b, int64_t n) {
+  const int64_t q = n / kNumXCD, r = n % kNumXCD;
+  const int64_t x = b % kNumXCD, k = b / kNumXCD;
+  return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+}
+
+// (x, y, z) block coordinates after the XCD remap of the whole grid
+struct BlockIdx3 {
+  int x, y, z;
+};
+__device__ __forceinline__ BlockIdx3 remapped_block() {
+  const int64_t nx = gridDim.x, ny = gridDim.y;
+  const int64_t n = nx * ny * gridDim.z;
+  const int64_t b = blockIdx.x + nx * (blockIdx.y + ny * (int64_t)blockIdx.z);
+  const int64_t l = xcd_remap(b, n);
+  BlockIdx3 r;
+  r.x = (int)(l % nx);
+  r.y = (int)((l / nx) % ny);
+  r.z = (int)(l / (nx * ny));
+  return r;
+}
+
 // device helper: the set owning position (original index or sorted
 // position) i, by bisection of set_off[0..nsets]
 __device__ __forceinline__ int find_set(const int32_t* set_off, int nsets, int64_t i) {
@@ -245,6 +274,11 @@ int grid_to_img(const hipnufft_plan_s* p, float2* img, const float2* smaps,
 // rows independent FP64 dots of a[rows][n] and b[rows][n] -> out[rows][2]
 // grid[t] *= K (elementwise complex) for the T grids of the plan workspace
 int mul_spectrum(const hipnufft_plan_s* p, const float2* K, int T);
+// Toeplitz convolution of chunk transforms [t0, t0+T) (an FFT-only plan whose
+// grid is 2N): expand / pad, FFT, multiply by K, inverse FFT, crop / combine
+// (forward sign -1; K includes the inverse's 1/prod(2N))
+int gram_chain(const hipnufft_plan_s* p, const float2* img, const float2* smaps, float2* out,
+               const float2* K, int C, int t0, int T);
 
 // FFT-only plan (no points): images of nmodes embedded, centred, in grids of
 // nf >= nmodes per axis, unit deconvolution (the Toeplitz operator's FFTs)

@@ -80,7 +80,17 @@ struct InterpArgs {
   int tg0, G;       // transforms [tg0, tg0 + G) of each set (gradient groups)
   float scale;
   int mode;
+  // staging of the tile + halo region: magic multipliers ceil(2^32 / R) for
+  // the region extent per axis and the region size (exact divisions of the
+  // cell index by umulhi), and whether one conditional wrap per axis suffices
+  uint32_t rmag[3];
+  uint32_t rcmag;
+  int wrap1;
 };
+
+__device__ __forceinline__ int mdiv(int c, uint32_t magic) {
+  return (int)__umulhi((uint32_t)c, magic);
+}
 
 // offset of (set k, transform t_local, original index i) in a k-space array
 // laid out per set as [ts][M_k]
@@ -321,8 +331,9 @@ __device__ __forceinline__ void stage_region(const InterpArgs& a, float2* lds, i
     rorg[d] = tc[d] * a.g.tile[d] - W / 2;
     RC *= R[d];
   }
-  // kStageBatch loads per thread are in flight before their LDS stores
-  constexpr int kStageBatch = 8;
+  // kStageBatch loads per thread are in flight before their LDS stores; the
+  // cell index is split by multiply-high divisions (c < 2^16, so exact)
+  constexpr int kStageBatch = 24;
   const int total = G * RC;
   for (int c0 = threadIdx.x; c0 < total; c0 += kStageBatch * kThreads) {
     float2 v[kStageBatch];
@@ -330,21 +341,28 @@ __device__ __forceinline__ void stage_region(const InterpArgs& a, float2* lds, i
     for (int u = 0; u < kStageBatch; ++u) {
       const int c = c0 + u * kThreads;
       if (c < total) {
-        const int gg = c / RC;
+        const int gg = G > 1 ? mdiv(c, a.rcmag) : 0;
         int r = c - gg * RC;
         int l[3];
 #pragma unroll
         for (int d = D - 1; d >= 0; --d) {
-          l[d] = r % R[d];
-          r /= R[d];
+          const int q = d > 0 ? mdiv(r, a.rmag[d]) : 0;
+          l[d] = d > 0 ? r - q * R[d] : r;
+          r = q;
         }
         int64_t gi = 0;
 #pragma unroll
         for (int d = 0; d < D; ++d) {
-          // the region can exceed the grid several times over (tiny grids)
-          int x = (rorg[d] + l[d]) % a.g.nf[d];
-          if (x < 0) x += a.g.nf[d];
-          gi = gi * a.g.nf[d] + x;
+          const int nfd = a.g.nf[d];
+          int x = rorg[d] + l[d];
+          if (a.wrap1) {  // region inside [-nf, 2nf)
+            if (x < 0) x += nfd;
+            if (x >= nfd) x -= nfd;
+          } else {  // the region can exceed the grid several times over (tiny grids)
+            x %= nfd;
+            if (x < 0) x += nfd;
+          }
+          gi = gi * nfd + x;
         }
         v[u] = a.grid[(int64_t)(t0 + gg) * a.grid_stride + gi];
       }
@@ -627,6 +645,21 @@ static int tile_cells(const hipnufft_plan_s* p, int halo) {
   return c;
 }
 
+static uint32_t magic_of(int64_t d) {  // ceil(2^32 / d), d >= 2
+  return (uint32_t)((((uint64_t)1 << 32) + (uint64_t)d - 1) / (uint64_t)d);
+}
+
+static void set_region_magic(const hipnufft_plan_s* p, InterpArgs* a) {
+  const int o = 3 - p->ndim;
+  a->wrap1 = 1;
+  for (int d = 0; d < p->ndim; ++d) {
+    const int R = p->tile[o + d] + p->w;  // >= 3
+    a->rmag[d] = magic_of(R);
+    if (p->nf[o + d] < R) a->wrap1 = 0;
+  }
+  a->rcmag = magic_of(tile_cells(p, p->w));
+}
+
 static int check_range(const hipnufft_plan_s* p, SetRange r) {
   if (r.k0 < 0 || r.k1 > p->nsets || r.k0 >= r.k1 || r.ts < 1)
     return set_error(HIPNUFFT_ERR_ARG, "invalid point-set range");
@@ -767,6 +800,7 @@ int launch_interp(const hipnufft_plan_s* p, const float2* grid, int64_t grid_str
   a.ts = r.ts;
   a.scale = scale;
   a.mode = mode;
+  set_region_magic(p, &a);
   const size_t cell_bytes = sizeof(float2) * (size_t)tile_cells(p, p->w);
   if (cell_bytes > kMaxLds) return set_error(HIPNUFFT_ERR_ARG, "interp tile too large for LDS");
   int G = std::min(p->interp_group, r.ts);
@@ -802,6 +836,7 @@ int launch_grad_traj(const hipnufft_plan_s* p, const float2* grid, int64_t grid_
   a.scale = scale;
   a.k0 = r.k0;
   a.ts = r.ts;
+  set_region_magic(p, &a);
   const int o = 3 - p->ndim;
   for (int d = 0; d < p->ndim; ++d) a.dscale[d] = (float)((double)p->nf[o + d] / (2.0 * M_PI));
   const size_t cell_bytes = sizeof(float2) * (size_t)tile_cells(p, p->w);

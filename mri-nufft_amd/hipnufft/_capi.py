@@ -9,12 +9,14 @@ fallback: if the library is missing or the device is absent every call raises.
 from __future__ import annotations
 
 import ctypes
+import os
 import threading
 from pathlib import Path
 
 import torch  # noqa: F401  (must be imported before the library is loaded)
 
-LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libhipnufft.so"
+LIB_PATH = Path(os.environ.get("HIPNUFFT_LIBRARY")
+                or Path(__file__).resolve().parent / "_lib" / "libhipnufft.so")
 
 ERR_ARG = -1
 ERR_HIP = -2
