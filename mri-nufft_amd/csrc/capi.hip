@@ -374,6 +374,8 @@ int create_fft_plan(int ndim, const int64_t* nmodes, const int64_t* nf, int ntra
     p->N[off + d] = nmodes[d];
     p->nf[off + d] = nf[d];
   }
+  if (!rc && p->nf[0] * p->nf[1] * p->nf[2] >= ((int64_t)1 << 30))
+    rc = set_error(HIPNUFFT_ERR_ARG, "grid too large (at most 2^30 cells)");
   for (int d = 0; d < 3 && !rc; ++d) {
     std::vector<float> dv(p->N[d], 1.0f);
     if (hipMalloc((void**)&p->dec[d], sizeof(float) * dv.size()) != hipSuccess ||
@@ -486,6 +488,10 @@ int hipnufft_plan_create(int type, int ndim, const int64_t* nmodes, int iflag, i
       nf = next_smooth_even(std::max<int64_t>(nf, 2 * p->w));
     }
     p->nf[off + d] = nf;
+    if (p->nf[0] * p->nf[1] * p->nf[2] >= ((int64_t)1 << 30)) {
+      free_plan(p);
+      return set_error(HIPNUFFT_ERR_ARG, "fine grid too large (at most 2^30 cells per transform)");
+    }
     int t = o.tile[d] > 0 ? o.tile[d] : default_tile(ndim, d);
     t = floor_pow2(std::max(t, 1));
     t = std::min(t, floor_pow2(nf));

@@ -125,17 +125,23 @@ __device__ __forceinline__ void stockham_pass(float2* buf, int n, int lgL, int L
                                               uint32_t ns_magic, const float2* tw) {
   constexpr int MAXB = (kEPT + R - 1) / R;
   const int L = 1 << lgL;
+  // uniform (scalar) quantities; every LDS address below is one 24-bit
+  // multiply per butterfly plus uniform strides per radix leg
   const int nr = n / R;
   const int nbf = nr << lgL;
   const int tstep = n / (Ns * R);
+  const int in_stride = nr * LP;
+  const int out_stride = Ns * LP;
+  const int NsR = Ns * R;
   float2 v[MAXB][R];
 #pragma unroll
   for (int i = 0; i < MAXB; ++i) {
     const int b = threadIdx.x + i * kThreads;
     if (b < nbf) {
       const int l = b & (L - 1), j = b >> lgL;
+      const int a0 = __mul24(j, LP) + l;
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[i][r] = buf[(j + r * nr) * LP + l];
+      for (int r = 0; r < R; ++r) v[i][r] = buf[a0 + r * in_stride];
     }
   }
   __syncthreads();
@@ -146,19 +152,20 @@ __device__ __forceinline__ void stockham_pass(float2* buf, int n, int lgL, int L
       const int l = b & (L - 1), j = b >> lgL;
       // q = j / Ns by multiply-high (exact for j < 2^16); Ns = 1 in the first pass
       const int q = Ns == 1 ? j : (int)__umulhi((uint32_t)j, ns_magic);
-      const int k = j - q * Ns;
+      const int k = j - __mul24(q, Ns);
       if (Ns > 1) {  // the first pass has unit twiddles
+        const int kt = __mul24(k, tstep);
 #pragma unroll
         for (int r = 1; r < R; ++r) {
-          float2 w = tw[r * k * tstep];
+          float2 w = tw[r * kt];
           if (S > 0) w.y = -w.y;
           v[i][r] = cmul(v[i][r], w);
         }
       }
       dftR<R, S>(v[i]);
-      const int base = q * Ns * R + k;
+      const int o0 = __mul24(__mul24(q, NsR) + k, LP) + l;
 #pragma unroll
-      for (int r = 0; r < R; ++r) buf[(base + r * Ns) * LP + l] = v[i][r];
+      for (int r = 0; r < R; ++r) buf[o0 + r * out_stride] = v[i][r];
     }
   }
   __syncthreads();
@@ -229,57 +236,57 @@ __global__ __launch_bounds__(kThreads) void k_rows_fwd(RowsArgs a) {
   const int t = bi.y;
   const int tt = a.t0 + t;
   const int64_t N = (int64_t)rows * a.M2;
-  const float2* src = a.img + (int64_t)(a.smaps ? tt / a.C : tt) * N;
-  const float2* sm = a.smaps ? a.smaps + (int64_t)(a.smaps_all ? tt : tt % a.C) * N : nullptr;
   const int n = a.nf2;
+  // line-major thread mapping: thread (l, x0) handles positions x0 + j * TPL
+  // of row r0 + l (n <= TPL * kEPT), so it keeps one row base and one scale
+  const int TPL = kThreads >> a.lgL;
+  const int l = threadIdx.x / TPL, x0 = threadIdx.x - l * TPL;
+  const int r = r0 + l;
+  const bool rok = r < rows;
+  const int i0 = rok ? r / a.M1 : 0, i1 = rok ? r - i0 * a.M1 : 0;
+  const float2* src = a.img + (int64_t)(a.smaps ? tt / a.C : tt) * N + (int64_t)r * a.M2;
+  const float2* sm =
+      a.smaps ? a.smaps + (int64_t)(a.smaps_all ? tt : tt % a.C) * N + (int64_t)r * a.M2 : nullptr;
+  const float rs = rok ? a.dec0[i0] * a.dec1[i1] * a.scale : 0.f;
+  const int M2 = a.M2, h2 = M2 / 2;
 
   // loads first (overlapping), then the arithmetic and LDS stores
-  float2 v[kEPT], s[kEPT];
+  float2 v[kEPT], sv[kEPT];
 #pragma unroll
-  for (int i = 0; i < kEPT; ++i) {
-    const int idx = threadIdx.x + i * kThreads;
-    const int l = idx / a.M2, x = idx - l * a.M2;
-    const int r = r0 + l;
-    v[i] = make_float2(0.f, 0.f);
-    s[i] = make_float2(1.f, 0.f);
-    if (idx < a.M2 * L && r < rows) {
-      const int64_t off = (int64_t)r * a.M2 + x;
-      v[i] = src[off];
-      if (sm) s[i] = sm[off];
+  for (int j = 0; j < kEPT; ++j) {
+    const int x = x0 + j * TPL;
+    v[j] = make_float2(0.f, 0.f);
+    sv[j] = make_float2(1.f, 0.f);
+    if (rok && x < M2) {
+      v[j] = src[x];
+      if (sm) sv[j] = sm[x];
     }
   }
 #pragma unroll
-  for (int i = 0; i < kEPT; ++i) {
-    const int idx = threadIdx.x + i * kThreads;
-    if (idx < a.M2 * L) {
-      const int l = idx / a.M2, x = idx - l * a.M2;
-      const int r = r0 + l;
-      float2 w = make_float2(0.f, 0.f);
-      if (r < rows) {
-        const int i0 = r / a.M1, i1 = r - i0 * a.M1;
-        float2 sv = s[i];
-        if (a.conj_smaps) sv.y = -sv.y;
-        w = cscale(cmul(v[i], sv), a.dec0[i0] * a.dec1[i1] * a.dec2[x] * a.scale);
-      }
-      buf[kept_pos(x, a.M2, n) * LP + l] = w;
+  for (int j = 0; j < kEPT; ++j) {
+    const int x = x0 + j * TPL;
+    if (x < M2) {
+      float2 s2 = sv[j];
+      if (a.conj_smaps) s2.y = -s2.y;
+      const float2 w = cscale(cmul(v[j], s2), rs * a.dec2[x]);
+      buf[(x < h2 ? x - h2 + n : x - h2) * LP + l] = w;
     }
   }
-  const int gap0 = a.M2 - a.M2 / 2, gap = n - a.M2;
-  for (int idx = threadIdx.x; idx < gap * L; idx += kThreads) {
-    const int l = idx & (L - 1), e = idx >> a.lgL;
-    buf[(gap0 + e) * LP + l] = make_float2(0.f, 0.f);
+  const int gap0 = M2 - h2, gap = n - M2;
+#pragma unroll
+  for (int j = 0; j < kEPT; ++j) {
+    const int e = x0 + j * TPL;
+    if (e < gap) buf[(gap0 + e) * LP + l] = make_float2(0.f, 0.f);
   }
   __syncthreads();
   fft_lines<S>(buf, twl, a.ax, a.lgL, LP);
-  float2* g = a.grid + (int64_t)t * a.gstride;
-  for (int idx = threadIdx.x; idx < n * L; idx += kThreads) {
-    const int l = idx / n, x = idx - l * n;
-    const int r = r0 + l;
-    if (r < rows) {
-      const int i0 = r / a.M1, i1 = r - i0 * a.M1;
-      const int64_t grow = ((int64_t)kept_pos(i0, a.M0, a.nf0) * a.nf1 + kept_pos(i1, a.M1, a.nf1)) * n;
-      g[grow + x] = buf[x * LP + l];
-    }
+  if (!rok) return;
+  float2* g = a.grid + (int64_t)t * a.gstride +
+              ((int64_t)kept_pos(i0, a.M0, a.nf0) * a.nf1 + kept_pos(i1, a.M1, a.nf1)) * n;
+#pragma unroll
+  for (int j = 0; j < kEPT; ++j) {
+    const int x = x0 + j * TPL;
+    if (x < n) g[x] = buf[x * LP + l];
   }
 }
 
@@ -405,65 +412,73 @@ __global__ __launch_bounds__(kThreads) void k_fft_strided(StridedArgs a) {
   const int oi = bi.y;
   const int t = bi.z;
   const int opos = a.outer_M ? kept_pos(oi, a.outer_M, a.outer_nf) : oi;
-  float2* base = a.data + (int64_t)t * a.tstride + (int64_t)opos * a.ostride + x0;
+  // thread -> (line l, positions e0 + i*E); a grid holds < 2^31 cells, so
+  // position offsets are 32-bit and advance by uniform steps per slot
+  const int E = kThreads >> a.lgL;
+  const int l = threadIdx.x & (L - 1), e0 = threadIdx.x >> a.lgL;
+  const bool lok = x0 + l < a.nx;
+  float2* base = a.data + (int64_t)t * a.tstride + (int64_t)opos * a.ostride + x0 + l;
+  const int ls = (int)a.lstride;
   const int n = a.n;
+  const int nls = n * ls, nLP = n * LP;
   const int nin = a.in_M ? a.in_M : n;
+  const int hin = a.in_M ? a.in_M / 2 : 0;  // kept position of e: e - h (+ n if e < h)
+  const int gofs = (e0 - hin) * ls, lofs = (e0 - hin) * LP + l;
   // all of a thread's loads are issued before the first LDS store, so they
   // overlap instead of paying one memory latency each (n * L <= kEPT * kThreads)
   float2 v[kEPT];
 #pragma unroll
   for (int i = 0; i < kEPT; ++i) {
-    const int idx = threadIdx.x + i * kThreads;
-    const int l = idx & (L - 1), e = idx >> a.lgL;
+    const int e = e0 + i * E;
     v[i] = make_float2(0.f, 0.f);
-    if (idx < nin * L && x0 + l < a.nx) {
-      const int pos = a.in_M ? kept_pos(e, a.in_M, n) : e;
-      v[i] = base[(int64_t)pos * a.lstride + l];
-    }
+    if (e < nin && lok) v[i] = base[gofs + i * (E * ls) + (e < hin ? nls : 0)];
   }
 #pragma unroll
   for (int i = 0; i < kEPT; ++i) {
-    const int idx = threadIdx.x + i * kThreads;
-    if (idx < nin * L) {
-      const int l = idx & (L - 1), e = idx >> a.lgL;
-      const int pos = a.in_M ? kept_pos(e, a.in_M, n) : e;
-      buf[pos * LP + l] = v[i];
-    }
+    const int e = e0 + i * E;
+    if (e < nin) buf[lofs + i * (E * LP) + (e < hin ? nLP : 0)] = v[i];
   }
-  if (a.in_M) {
-    const int gap0 = a.in_M - a.in_M / 2, gap = n - a.in_M;
-    for (int idx = threadIdx.x; idx < gap * L; idx += kThreads) {
-      const int l = idx & (L - 1), e = idx >> a.lgL;
-      buf[(gap0 + e) * LP + l] = make_float2(0.f, 0.f);
+  if (a.in_M) {  // zero the positions between the kept halves
+    const int gap0 = a.in_M - hin, gap = n - a.in_M;
+#pragma unroll
+    for (int i = 0; i < kEPT; ++i) {
+      const int e = e0 + i * E;
+      if (e < gap) buf[(gap0 + e) * LP + l] = make_float2(0.f, 0.f);
     }
   }
   __syncthreads();
   fft_lines<S>(buf, twl, a.ax, a.lgL, LP);
   if constexpr (CONV) {
-    const float2* kb = a.K + (int64_t)opos * a.ostride + x0;
+    const float2* kb = a.K + (int64_t)opos * a.ostride + x0 + l;
+    const int kofs = e0 * ls;
 #pragma unroll
     for (int i = 0; i < kEPT; ++i) {
-      const int idx = threadIdx.x + i * kThreads;
-      const int l = idx & (L - 1), e = idx >> a.lgL;
+      const int e = e0 + i * E;
       v[i] = make_float2(0.f, 0.f);
-      if (idx < n * L && x0 + l < a.nx) v[i] = kb[(int64_t)e * a.lstride + l];
+      if (e < n && lok) v[i] = kb[kofs + i * (E * ls)];
     }
 #pragma unroll
     for (int i = 0; i < kEPT; ++i) {
-      const int idx = threadIdx.x + i * kThreads;
-      if (idx < n * L) {
-        const int l = idx & (L - 1), e = idx >> a.lgL;
-        buf[e * LP + l] = cmul(buf[e * LP + l], v[i]);
+      const int e = e0 + i * E;
+      if (e < n) {
+        const int q = e * LP + l;
+        buf[q] = cmul(buf[q], v[i]);
       }
     }
     __syncthreads();
     fft_lines<-S>(buf, twl, a.ax, a.lgL, LP);
   }
   const int nout = a.out_M ? a.out_M : n;
-  for (int idx = threadIdx.x; idx < nout * L; idx += kThreads) {
-    const int l = idx & (L - 1), e = idx >> a.lgL;
-    const int pos = a.out_M ? kept_pos(e, a.out_M, n) : e;
-    if (x0 + l < a.nx) base[(int64_t)pos * a.lstride + l] = buf[pos * LP + l];
+  const int hout = a.out_M ? a.out_M / 2 : 0;
+  const int gofs2 = (e0 - hout) * ls, lofs2 = (e0 - hout) * LP + l;
+  if (lok) {
+#pragma unroll
+    for (int i = 0; i < kEPT; ++i) {
+      const int e = e0 + i * E;
+      if (e < nout)
+        base[gofs2 + i * (E * ls) + (e < hout ? nls : 0)] =
+            buf[lofs2 + i * (E * LP) + (e < hout ? nLP : 0)];
+    }
   }
 }
 
