@@ -274,6 +274,133 @@ __global__ __launch_bounds__(kThreads) void k_spread(SpreadArgs a) {
   }
 }
 
+// Column-owner spreading for 3D tiles of 8 x 8 x 16 cells and 16 transforms
+// per workgroup: lane c of every wave owns tile column c (16 cells along
+// the contiguous axis), wave v owns transforms [4v, 4v + 4).  Per staged
+// point the column weight is formed once per lane, and the point's cells
+// along the column are found from its (wave-uniform, scalar) footprint start,
+// so a lane only does arithmetic for the 5..7 cells the footprint covers.
+// Same list order, same summation order per cell as k_spread: deterministic.
+constexpr int kColsZ = 16;       // cells per column
+constexpr int kColsCW = 4;       // transforms per wave
+constexpr int kColsChunk = 128;  // staged points per round
+
+__global__ __launch_bounds__(kThreads) void k_spread_cols(SpreadArgs a) {
+  extern __shared__ float smem[];
+  constexpr int D = 3, GW = 4 * kColsCW;  // transforms per workgroup
+  const int W = a.g.w;
+  const int PW = D + D * W + 2 * GW;
+  const int item = a.item0 + blockIdx.x;
+  const int key = a.it_tile[item];
+  const int set = key / a.g.nbins;
+  const int tile = key - set * a.g.nbins;
+  const int slot = a.it_slot[item];
+  const int tg = blockIdx.y * GW;        // first transform of the set handled here
+  const int tb = (set - a.k0) * a.ts;    // chunk transform of the set's first
+  const int Gv = min(GW, a.ts - tg);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int cw = wave * kColsCW;         // this wave's first transform (of the GW)
+  const float inv_hw = 2.0f / W, beta = a.g.beta;
+
+  int tc[3], org[3], ext[3], nf[3];
+  tile_coords<D>(tile, a.g.nt, tc);
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    nf[d] = a.g.nf[d];
+    org[d] = tc[d] * a.g.tile[d];
+    ext[d] = min(a.g.tile[d], nf[d] - org[d]);
+  }
+  const int c0 = lane >> 3, c1 = lane & 7;
+  const bool colown = c0 < ext[0] && c1 < ext[1];
+  const int zext = ext[2];
+
+  float2 acc[kColsZ][kColsCW];
+#pragma unroll
+  for (int z = 0; z < kColsZ; ++z)
+#pragma unroll
+    for (int g = 0; g < kColsCW; ++g) acc[z][g] = make_float2(0.f, 0.f);
+
+  const int e0 = a.it_begin[item];
+  const int e1 = min(e0 + a.smax, a.list_start[key + 1]);
+  for (int eb = e0; eb < e1; eb += kColsChunk) {
+    const int n = min(kColsChunk, e1 - eb);
+    if ((int)threadIdx.x < n) {
+      float* sp = smem + threadIdx.x * PW;
+      const int p = a.list_pts[eb + threadIdx.x];
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        const float off = a.g.off[d][p];
+        reinterpret_cast<int*>(sp)[d] = a.g.cell[d][p] - org[d];  // raw start, mod nf
+        for (int j = 0; j < W; ++j) sp[D + d * W + j] = es_eval(off - (float)j, inv_hw, beta);
+      }
+      float* sv = sp + D + D * W;
+      for (int g = 0; g < GW; ++g) {
+        float2 v = make_float2(0.f, 0.f);
+        if (g < Gv) v = a.in[(int64_t)(tg + g) * a.in_stride + p];
+        sv[2 * g] = v.x;
+        sv[2 * g + 1] = v.y;
+      }
+    }
+    __syncthreads();
+    if (cw < Gv) {  // uniform per wave
+      for (int q = 0; q < n; ++q) {
+        const float* sp = smem + q * PW;
+        const int* si = reinterpret_cast<const int*>(sp);
+        // column weight (per lane)
+        int d0 = c0 - si[0], d1 = c1 - si[1];
+        if (d0 < 0) d0 += nf[0];
+        if (d0 >= nf[0]) d0 -= nf[0];
+        if (d1 < 0) d1 += nf[1];
+        if (d1 >= nf[1]) d1 -= nf[1];
+        const bool hc = colown && d0 < W && d1 < W;
+        const float wc = hc ? sp[D + d0] * sp[D + W + d1] : 0.0f;
+        // footprint start along the column: the same for every lane
+        const int s2 = __builtin_amdgcn_readfirstlane(si[2]);
+        const float* w2 = sp + D + 2 * W;
+        const float* sv = sp + D + D * W + 2 * cw;
+        float2 str[kColsCW];
+#pragma unroll
+        for (int g = 0; g < kColsCW; ++g) str[g] = make_float2(sv[2 * g], sv[2 * g + 1]);
+#pragma unroll
+        for (int z = 0; z < kColsZ; ++z) {
+          int da = z - s2;  // scalar
+          if (da < 0) da += nf[2];
+          if (da >= nf[2]) da -= nf[2];
+          if (z < zext && da < W) {
+            const float f = wc * w2[da];
+#pragma unroll
+            for (int g = 0; g < kColsCW; ++g) {
+              acc[z][g].x += f * str[g].x;
+              acc[z][g].y += f * str[g].y;
+            }
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+
+  if (!colown) return;
+  const int TC = a.g.tile[0] * a.g.tile[1] * a.g.tile[2];
+  const int64_t gcol = (((int64_t)(org[0] + c0) * nf[1]) + org[1] + c1) * nf[2] + org[2];
+#pragma unroll
+  for (int g = 0; g < kColsCW; ++g) {
+    if (cw + g >= Gv) break;  // uniform
+    const int t = tb + tg + cw + g;
+    if (slot >= 0) {  // partial tile of a dense tile: summed by k_spread_reduce
+      float2* dst = a.part + ((int64_t)slot * a.ntrans + t) * TC + lane * kColsZ;
+#pragma unroll
+      for (int z = 0; z < kColsZ; ++z)
+        if (z < zext) dst[z] = acc[z][g];
+    } else {
+      float2* dst = a.grid + (int64_t)t * a.grid_stride + gcol;
+#pragma unroll
+      for (int z = 0; z < kColsZ; ++z)
+        if (z < zext) dst[z] = acc[z][g];
+    }
+  }
+}
+
 // Sum the partial tiles of each dense tile in slot order (deterministic).
 template <int D>
 __global__ __launch_bounds__(kThreads) void k_spread_reduce(SpreadArgs a) {
@@ -742,8 +869,26 @@ int launch_spread_sorted(const hipnufft_plan_s* p, const float2* in, SetRange r,
     if (ncol * ((zl + kZ - 1) / kZ) > kThreads)
       return set_error(HIPNUFFT_ERR_ARG, "spread tile has more than 256 column segments of 4 cells");
   }
-  int G = p->spread_group;
-  G = G >= 16 ? 16 : G >= 8 ? 8 : G >= 4 ? 4 : G >= 2 ? 2 : 1;
+  // column owners for 3D 8 x 8 x 16 tiles when asked for (spread_group >= 16
+  // and at least 8 transforms per set); otherwise the cell-owner kernel
+  if (p->ndim == 3 && p->tile[0] == 8 && p->tile[1] == 8 && p->tile[2] == kColsZ &&
+      r.ts >= 8 && p->spread_group >= 16) {
+    const size_t lds = sizeof(float) * (size_t)kColsChunk * (3 + 3 * p->w + 2 * 4 * kColsCW);
+    if (nitems > 0) {
+      const dim3 grd((unsigned)nitems, (unsigned)((r.ts + 4 * kColsCW - 1) / (4 * kColsCW)));
+      StageScope timer(p, kStageSpread);
+      hipLaunchKernelGGL(k_spread_cols, grd, dim3(kThreads), lds, p->stream, a);
+      HN_LAUNCH_CHECK();
+    }
+    if (ndense > 0) {
+      const dim3 rg((unsigned)ndense, (unsigned)r.ts);
+      hipLaunchKernelGGL(k_spread_reduce<3>, rg, dim3(kThreads), 0, p->stream, a);
+      HN_LAUNCH_CHECK();
+    }
+    return 0;
+  }
+  int G = p->spread_group > 0 ? p->spread_group : 8;
+  G = G >= 8 ? 8 : G >= 4 ? 4 : G >= 2 ? 2 : 1;
   while (G > 1 && G / 2 >= r.ts) G /= 2;
   auto lds_of = [&](int g) {
     return sizeof(float) * (size_t)kSpreadChunk * (p->ndim * p->w + p->ndim + 2 * g);
@@ -756,7 +901,6 @@ int launch_spread_sorted(const hipnufft_plan_s* p, const float2* in, SetRange r,
     StageScope timer(p, kStageSpread);
 #define HN_SPREAD(DD)                                                                        \
   switch (G) {                                                                               \
-    case 16: hipLaunchKernelGGL((k_spread<DD, 16>), grd, dim3(kThreads), lds, p->stream, a); break; \
     case 8: hipLaunchKernelGGL((k_spread<DD, 8>), grd, dim3(kThreads), lds, p->stream, a); break; \
     case 4: hipLaunchKernelGGL((k_spread<DD, 4>), grd, dim3(kThreads), lds, p->stream, a); break; \
     case 2: hipLaunchKernelGGL((k_spread<DD, 2>), grd, dim3(kThreads), lds, p->stream, a); break; \
