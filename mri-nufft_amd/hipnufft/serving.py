@@ -15,7 +15,10 @@ Every CG iteration after the first two is a replay of a HIP graph captured
 per slot, and the stopping test is a device flag (as in
 ``pinv_solver(optim="cg", graph=True)``), so a request gives the same result
 alone and among others: same kernels, same summation order, and a Lipschitz
-constant computed from a seeded start vector.
+constant computed from a seeded start vector.  By default each resident plan
+also holds the Toeplitz form of its Gram operator, so the CG gradients
+``A^H (A x - y)`` are evaluated as ``G x - A^H y`` with FFTs only (``A^H y``
+once per request); ``toeplitz=False`` uses the fused NUFFT data consistency.
 
 The reference has no serving layer; each reconstruction there is a
 ``pinv_solver`` call on a freshly built operator (``base.py:667-690``,
@@ -65,7 +68,7 @@ class ReconResult:
         return self.finished - self.arrival
 
 
-def estimate_bytes(shape, n_samples, n_coils, upsampfac=2.0):
+def estimate_bytes(shape, n_samples, n_coils, upsampfac=2.0, toeplitz=False):
     """Device memory of one request's plan, data and CG state (bytes)."""
     N = float(np.prod(shape))
     nf = float(np.prod([max(int(upsampfac * n), 16) for n in shape]))
@@ -74,21 +77,36 @@ def estimate_bytes(shape, n_samples, n_coils, upsampfac=2.0):
     pts = n_samples * (4 + 8 * d) + 4.5 * n_samples * 4 * (2 if d == 3 else 1)
     ksp = 3 * n_coils * n_samples * 8          # observations + two sorted workspaces
     img = (n_coils + 6) * N * 8                 # coil maps + CG vectors
-    return int(grid + pts + ksp + img + (8 << 20))
+    gram = (n_coils + 1) * (2 ** d) * N * 8 if toeplitz else 0  # workspace + spectrum
+    return int(grid + pts + ksp + img + gram + (8 << 20))
+
+
+def toeplitz_supported(shape):
+    """The Toeplitz form needs 2N to be 2,3,5-smooth on every axis (and 4N
+    within the FFT length limit for its kernel build)."""
+    def smooth(n):
+        for f in (2, 3, 5):
+            while n % f == 0:
+                n //= f
+        return n == 1
+    return all(smooth(2 * int(n)) and 4 * int(n) <= 2730 for n in shape)
 
 
 class _Plan:
     """A resident operator for one trajectory (plan, sorted points, L)."""
 
-    def __init__(self, key, samples, shape, n_coils, stream, seed):
+    def __init__(self, key, samples, shape, n_coils, stream, seed, toeplitz):
         with torch.cuda.stream(stream):
             self.op = MRIHipNUFFT(samples, shape, n_coils=n_coils, smaps=None,
                                   squeeze_dims=False)
             self.op._smaps = torch.zeros((n_coils, *shape), dtype=torch.complex64,
                                          device=self.op.device)
             self.lipschitz = float(self.op.get_lipschitz_cst(seed=seed))
+            self.toeplitz = bool(toeplitz) and toeplitz_supported(shape)
+            if self.toeplitz:
+                self.op.compute_toeplitz_kernel()
         self.key = key
-        self.bytes = estimate_bytes(shape, self.op.n_samples, n_coils)
+        self.bytes = estimate_bytes(shape, self.op.n_samples, n_coils, toeplitz=self.toeplitz)
         self.in_use = False
 
 
@@ -107,8 +125,10 @@ class _CGJob:
         inv_l = 1.0 / plan.lipschitz
         with torch.cuda.stream(stream):
             self.y = y
+            # Toeplitz plans: A^H (A x - y) = G x - A^H y with A^H y formed once
+            self.rhs = self.op.adj_op(y).reshape(shape) if plan.toeplitz else None
             image = torch.zeros(shape, dtype=torch.complex64, device=dev)
-            grad = self.op.data_consistency(image, y).reshape(shape)
+            grad = self._dc(image)
             vel = grad / plan.lipschitz
             image = image - vel
             self.st = dict(image=image.contiguous(), grad=grad.contiguous(), vel=vel.contiguous(),
@@ -123,9 +143,14 @@ class _CGJob:
         self.graph = None
         self.event = None
 
+    def _dc(self, image):
+        if self.rhs is not None:
+            return self.op.gram_op(image).reshape(self.shape) - self.rhs
+        return self.op.data_consistency(image, self.y).reshape(self.shape)
+
     def _step(self):
         st, bufs, shape = self.st, self.bufs, self.shape
-        gn = self.op.data_consistency(st["image"], self.y).reshape(shape)
+        gn = self._dc(st["image"])
         ws = bufs["ws"]
         _capi.dot_c64(gn.reshape(-1), gn.reshape(-1), True, out=bufs["nrm"], ws=ws)
         diff = gn - st["grad"]
@@ -191,9 +216,11 @@ class _CGJob:
 class ReconServer:
     """Slot-based continuous batching under an HBM budget (see module doc)."""
 
-    def __init__(self, hbm_budget_bytes=64 << 30, max_slots=8, use_graph=True, seed=1234):
+    def __init__(self, hbm_budget_bytes=64 << 30, max_slots=8, use_graph=True, seed=1234,
+                 toeplitz=True):
         if not MRIHipNUFFT.available:
             raise RuntimeError("hipnufft is not available (library or GPU missing)")
+        self.toeplitz = bool(toeplitz)
         self.budget = int(hbm_budget_bytes)
         self.max_slots = int(max_slots)
         self.use_graph = bool(use_graph)
@@ -244,7 +271,8 @@ class ReconServer:
             plan = None
             key = (key, req.rid)  # a second resident copy for a concurrent request
         if plan is None:
-            need = estimate_bytes(req.shape, req.kspace.shape[1], C)
+            need = estimate_bytes(req.shape, req.kspace.shape[1], C,
+                                  toeplitz=self.toeplitz and toeplitz_supported(req.shape))
             if need > self.budget:
                 raise ValueError(f"request {req.rid} needs {need} bytes, over the HBM budget")
             if not self._make_room(need):
@@ -253,7 +281,7 @@ class ReconServer:
         stream = self.streams[si]
         if plan is None:
             traj = proper_trajectory(req.samples, normalize="pi")
-            plan = _Plan(key, traj, tuple(req.shape), C, stream, self.seed)
+            plan = _Plan(key, traj, tuple(req.shape), C, stream, self.seed, self.toeplitz)
             self.plans[key] = plan
             self.stats["plan_builds"] += 1
         else:

@@ -217,3 +217,11 @@ def test_autotune_candidates_respect_kernel_limits():
             cands = tile_candidates(nd, w)
             assert cands and all(len(c) == nd and tile_ok(c, w) for c in cands)
             assert all(t & (t - 1) == 0 for c in cands for t in c)  # powers of two
+
+
+def test_toeplitz_support_rule():
+    from hipnufft.serving import toeplitz_supported
+
+    assert toeplitz_supported((192, 192, 192)) and toeplitz_supported((48, 48))
+    assert not toeplitz_supported((40, 56))      # 112 = 2^4 * 7
+    assert not toeplitz_supported((1024, 1024))  # kernel build grid 4096 > 2730

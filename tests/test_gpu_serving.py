@@ -31,23 +31,28 @@ def _requests():
     return reqs
 
 
-def test_alone_vs_mix_identical():
+@pytest.mark.parametrize("toeplitz", [True, False])
+def test_alone_vs_mix_identical(toeplitz):
     from hipnufft.serving import ReconServer
 
     reqs = _requests()
-    mix = ReconServer(max_slots=4).run(reqs, realtime=False)
+    mix = ReconServer(max_slots=4, toeplitz=toeplitz).run(reqs, realtime=False)
     assert sorted(mix) == list(range(len(reqs)))
     for r in reqs:
-        alone = ReconServer(max_slots=1).run([r], realtime=False)[r.rid]
+        alone = ReconServer(max_slots=1, toeplitz=toeplitz).run([r], realtime=False)[r.rid]
         assert rel_l2(mix[r.rid].image, alone.image) <= 1e-6, r.rid
 
 
-def test_served_matches_plain_cg():
+@pytest.mark.parametrize("toeplitz,tol", [(False, 1e-5), (True, 1e-4)])
+def test_served_matches_plain_cg(toeplitz, tol):
+    """Served results against the plain per-call CG with the same L (the
+    Toeplitz gradients differ from the NUFFT ones at the NUFFT tolerance;
+    the stated reconstruction bar is 1e-4)."""
     from hipnufft import MRIHipNUFFT, cg
     from hipnufft.serving import ReconServer
 
     reqs = _requests()[:4]
-    srv = ReconServer(max_slots=4)
+    srv = ReconServer(max_slots=4, toeplitz=toeplitz)
     res = srv.run(reqs, realtime=False)
     for r in reqs:
         op = MRIHipNUFFT(r.samples, r.shape, n_coils=r.kspace.shape[0], smaps=r.smaps,
@@ -57,14 +62,15 @@ def test_served_matches_plain_cg():
         op.get_lipschitz_cst = lambda **_: L
         y = torch.from_numpy(r.kspace).to(op.device).reshape(1, r.kspace.shape[0], -1)
         ref = cg(op, y, max_iter=r.max_iter, tol=r.tol).reshape(r.shape).cpu().numpy()
-        assert rel_l2(res[r.rid].image, ref) <= 1e-5, r.rid
+        assert rel_l2(res[r.rid].image, ref) <= tol, r.rid
 
 
 def test_budget_admission_and_reuse():
     from hipnufft.serving import ReconServer, estimate_bytes
 
     reqs = _requests()
-    big = max(estimate_bytes(r.shape, r.kspace.shape[1], r.kspace.shape[0]) for r in reqs)
+    big = max(estimate_bytes(r.shape, r.kspace.shape[1], r.kspace.shape[0], toeplitz=True)
+              for r in reqs)
     srv = ReconServer(hbm_budget_bytes=int(1.5 * big), max_slots=4)
     res = srv.run(reqs, realtime=False)
     free = ReconServer(max_slots=4).run(reqs, realtime=False)
