@@ -92,22 +92,46 @@ def toeplitz_supported(shape):
     return all(smooth(2 * int(n)) and 4 * int(n) <= 2730 for n in shape)
 
 
+def plan_ntrans(shape, n_coils, upsampfac=2.0, cap_bytes=8 << 30):
+    """Transforms per launch of a resident plan: enough for 16 coils (requests
+    with more are chunked), within ``cap_bytes`` of fine grids."""
+    cell = 8.0 * float(np.prod([max(int(upsampfac * n), 16) for n in shape]))
+    return int(max(1, min(max(n_coils, 16), cap_bytes // cell)))
+
+
+
 class _Plan:
-    """A resident operator for one trajectory (plan, sorted points, L)."""
+    """A resident operator for one trajectory and image shape: sorted points,
+    workspaces, the Lipschitz constant and the Toeplitz kernel, shared by every
+    request of that protocol whatever its coil count (the coil count and maps
+    are set per request)."""
 
     def __init__(self, key, samples, shape, n_coils, stream, seed, toeplitz):
+        nt = plan_ntrans(shape, n_coils)
         with torch.cuda.stream(stream):
-            self.op = MRIHipNUFFT(samples, shape, n_coils=n_coils, smaps=None,
+            # built for nt coils (its workspace holds nt transforms per launch);
+            # each request sets its own coil count, larger ones are chunked
+            self.op = MRIHipNUFFT(samples, shape, n_coils=max(nt, 1), smaps=None,
                                   squeeze_dims=False)
             self.op._smaps = torch.zeros((n_coils, *shape), dtype=torch.complex64,
                                          device=self.op.device)
+            self.op.n_coils = n_coils
             self.lipschitz = float(self.op.get_lipschitz_cst(seed=seed))
             self.toeplitz = bool(toeplitz) and toeplitz_supported(shape)
             if self.toeplitz:
                 self.op.compute_toeplitz_kernel()
         self.key = key
-        self.bytes = estimate_bytes(shape, self.op.n_samples, n_coils, toeplitz=self.toeplitz)
+        self.bytes = estimate_bytes(shape, self.op.n_samples, max(n_coils, self.op.n_trans),
+                                    toeplitz=self.toeplitz)
         self.in_use = False
+
+    def bind(self, req, stream):
+        """Point the resident operator at one request's coils and maps."""
+        C = int(req.kspace.shape[0])
+        op = self.op
+        with torch.cuda.stream(stream):
+            op.n_coils = C
+            op._smaps = torch.from_numpy(np.ascontiguousarray(req.smaps)).to(op.device)
 
 
 class _CGJob:
@@ -258,20 +282,24 @@ class ReconServer:
 
     def _key(self, req):
         if req.traj_key is not None:
-            return (req.traj_key, tuple(req.shape), req.kspace.shape[0])
-        return (id(req), tuple(req.shape), req.kspace.shape[0])
+            return (req.traj_key, tuple(req.shape))
+        return (("request", req.rid), tuple(req.shape))
 
     def _try_admit(self, req, now):
+        """Admit ``req`` if a slot and memory are free.  A request whose
+        resident plan is busy with another request waits for it (building a
+        second copy of a large plan costs more than the wait); other waiting
+        requests are admitted meanwhile."""
         if not self.free_streams:
             return False
         key = self._key(req)
         C = int(req.kspace.shape[0])
         plan = self.plans.get(key)
         if plan is not None and plan.in_use:
-            plan = None
-            key = (key, req.rid)  # a second resident copy for a concurrent request
+            return False
         if plan is None:
-            need = estimate_bytes(req.shape, req.kspace.shape[1], C,
+            need = estimate_bytes(req.shape, req.kspace.shape[1],
+                                  max(C, plan_ntrans(req.shape, C)),
                                   toeplitz=self.toeplitz and toeplitz_supported(req.shape))
             if need > self.budget:
                 raise ValueError(f"request {req.rid} needs {need} bytes, over the HBM budget")
@@ -288,9 +316,9 @@ class ReconServer:
             self.plans.move_to_end(key)
             self.stats["plan_hits"] += 1
         plan.in_use = True
+        plan.bind(req, stream)
         dev = plan.op.device
         with torch.cuda.stream(stream):
-            plan.op._smaps.copy_(torch.from_numpy(np.ascontiguousarray(req.smaps)).to(dev))
             y = torch.from_numpy(np.ascontiguousarray(req.kspace, dtype=np.complex64)).to(dev)
             y = y.reshape(1, C, -1)
             job = _CGJob(plan, y, req.max_iter, req.tol, stream)
@@ -323,8 +351,14 @@ class ReconServer:
             now = time.perf_counter() - t0
             while pending and (not realtime or pending[0].arrival <= now):
                 self.submit(pending.popleft())
-            while self.waiting and self._try_admit(self.waiting[0], now):
-                self.waiting.popleft()
+            if self.waiting and self.free_streams:
+                # oldest first; a request whose plan is busy is passed over
+                keep = deque()
+                while self.waiting:
+                    r = self.waiting.popleft()
+                    if not (self.free_streams and self._try_admit(r, now)):
+                        keep.append(r)
+                self.waiting = keep
             for _, job, _, _ in self.active:
                 job.advance(self.use_graph)
             self._retire(time.perf_counter() - t0)

@@ -66,11 +66,12 @@ def test_served_matches_plain_cg(toeplitz, tol):
 
 
 def test_budget_admission_and_reuse():
-    from hipnufft.serving import ReconServer, estimate_bytes
+    from hipnufft.serving import ReconServer, estimate_bytes, plan_ntrans
 
     reqs = _requests()
-    big = max(estimate_bytes(r.shape, r.kspace.shape[1], r.kspace.shape[0], toeplitz=True)
-              for r in reqs)
+    big = max(estimate_bytes(r.shape, r.kspace.shape[1],
+                             max(r.kspace.shape[0], plan_ntrans(r.shape, r.kspace.shape[0])),
+                             toeplitz=True) for r in reqs)
     srv = ReconServer(hbm_budget_bytes=int(1.5 * big), max_slots=4)
     res = srv.run(reqs, realtime=False)
     free = ReconServer(max_slots=4).run(reqs, realtime=False)
