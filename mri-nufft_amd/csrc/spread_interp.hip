@@ -81,11 +81,13 @@ struct InterpArgs {
   float scale;
   int mode;
   // staging of the tile + halo region: magic multipliers ceil(2^32 / R) for
-  // the region extent per axis and the region size (exact divisions of the
-  // cell index by umulhi), and whether one conditional wrap per axis suffices
+  // the region extent per axis (exact divisions by umulhi), whether one
+  // conditional wrap per axis suffices, lanes per staged row (the last-axis
+  // extent, at most the workgroup) and its magic multiplier, rows per pass
   uint32_t rmag[3];
-  uint32_t rcmag;
   int wrap1;
+  int stage_lanes, stage_rows;
+  uint32_t lane_mag;
 };
 
 __device__ __forceinline__ int mdiv(int c, uint32_t magic) {
@@ -274,6 +276,201 @@ __global__ __launch_bounds__(kThreads) void k_spread(SpreadArgs a) {
   }
 }
 
+// Block-owner spreading for 3D tiles with edges of 4, 8 or 16 cells (4, 8
+// or 16 blocks of 4 x 4 x 4 cells, at most 4 along an axis): lane l owns
+// cell (l / 16, l / 4 % 4, l % 4) of each of the wave's blocks (wave v owns
+// blocks v, v + 4, ...).  When an entry is staged, its thread records which
+// blocks of the tile the footprint touches (a bit mask) and, per axis and
+// block along it, where the lane weights of that block start in the entry's
+// weight rows (padded with 3 zeros on each side), so a wave skips untouched
+// blocks with one bit test and a touched block costs three weight reads
+// and the products per lane.  A cell outside the footprint adds f = 0 times
+// the strength: its sum is unchanged, and every cell's sum is formed in
+// list order with the products (w0 * w1) * w2 of the cell-owner kernel:
+// the same bits.
+constexpr int kBlkE = 4;            // block edge
+constexpr int kBlkPerWave = 4;      // at most 16 blocks per tile
+constexpr int kBlkChunk = 128;      // staged entries per round
+constexpr int kBlkPad = 3;          // zeros before / after each weight row
+constexpr int kBlkHead = 4;         // mask + one word of 4 byte offsets per axis
+
+// floats per staged entry of the block kernel: header (mask, 3 words of
+// byte offsets), G strengths, 3 padded weight rows; a multiple of 4 so that
+// every record starts 16-byte aligned
+__host__ __device__ constexpr int blk_record_floats(int W, int G) {
+  return (kBlkHead + 2 * G + 3 * (W + 2 * kBlkPad) + 3) & ~3;
+}
+
+template <int G>
+__global__ __launch_bounds__(kThreads) void k_spread_blocks(SpreadArgs a) {
+  extern __shared__ float smem[];
+  const int W = a.g.w;
+  const int WP = W + 2 * kBlkPad;
+  const int PW = blk_record_floats(W, G);
+  const int item = a.item0 + blockIdx.x;
+  const int key = a.it_tile[item];
+  const int set = key / a.g.nbins;
+  const int tile = key - set * a.g.nbins;
+  const int slot = a.it_slot[item];
+  const int t0 = blockIdx.y * G;
+  const int tb = (set - a.k0) * a.ts;
+  const int Gv = min(G, a.ts - t0);
+  const float inv_hw = 2.0f / W, beta = a.g.beta;
+
+  int tc[3], org[3], ext[3], td[3], nf[3], nb[3];
+  tile_coords<3>(tile, a.g.nt, tc);
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    td[d] = a.g.tile[d];
+    nf[d] = a.g.nf[d];
+    org[d] = tc[d] * td[d];
+    ext[d] = min(td[d], nf[d] - org[d]);
+    nb[d] = td[d] / kBlkE;
+  }
+  const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x / 64);
+  const int lane = (int)threadIdx.x & 63;
+  const int lc[3] = {lane >> 4, (lane >> 2) & 3, lane & 3};
+  const int nbw = nb[0] * nb[1] * nb[2] / 4;  // blocks per wave (uniform)
+  // the wave's blocks: index along each axis, mask bit, ownership of the
+  // lane's cell; a block wholly past the grid end is never touched
+  int bidx[kBlkPerWave][3];
+  uint32_t bbit[kBlkPerWave];
+  bool own[kBlkPerWave];
+#pragma unroll
+  for (int bi = 0; bi < kBlkPerWave; ++bi) {
+    const int b = wave + 4 * bi;
+    bidx[bi][2] = b % nb[2];
+    bidx[bi][1] = (b / nb[2]) % nb[1];
+    bidx[bi][0] = b / (nb[2] * nb[1]);
+    bool live = bi < nbw;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) live = live && bidx[bi][d] * kBlkE < ext[d];
+    bbit[bi] = live ? 1u << b : 0u;
+    own[bi] = live;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) own[bi] = own[bi] && bidx[bi][d] * kBlkE + lc[d] < ext[d];
+  }
+
+  float2 acc[kBlkPerWave][G];
+#pragma unroll
+  for (int bi = 0; bi < kBlkPerWave; ++bi)
+#pragma unroll
+    for (int g = 0; g < G; ++g) acc[bi][g] = make_float2(0.f, 0.f);
+
+  const int e0 = a.it_begin[item];
+  const int e1 = min(e0 + a.smax, a.list_start[key + 1]);
+  for (int eb = e0; eb < e1; eb += kBlkChunk) {
+    const int n = min(kBlkChunk, e1 - eb);
+    if ((int)threadIdx.x < n) {
+      float* sp = smem + threadIdx.x * PW;
+      const int p = a.list_pts[eb + threadIdx.x];
+      uint32_t hit[3] = {0u, 0u, 0u};  // per axis: blocks along it the footprint meets
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const float off = a.g.off[d][p];
+        const int st = a.g.cell[d][p] - org[d];  // raw start, mod nf
+        float* wr = sp + kBlkHead + 2 * G + d * WP;
+        for (int j = 0; j < kBlkPad; ++j) wr[j] = 0.0f;
+        for (int j = 0; j < W; ++j) wr[kBlkPad + j] = es_eval(off - (float)j, inv_hw, beta);
+        for (int j = 0; j < kBlkPad; ++j) wr[kBlkPad + W + j] = 0.0f;
+        // offset of the block's first cell from the footprint start, in
+        // [-kBlkPad, W) when the block meets the footprint (nf >= W +
+        // kBlkPad; a block inside the grid has org + its origin < nf, so
+        // one wrap each way suffices); past the grid end: not met
+        uint32_t packed = 0u;
+        for (int bd = 0; bd < nb[d]; ++bd) {
+          const int bo = bd * kBlkE;
+          int x = bo - st;
+          if (x < 0) x += nf[d];
+          if (x >= nf[d]) x -= nf[d];
+          if (x >= nf[d] - kBlkPad) x -= nf[d];
+          if (bo < ext[d] && x < W) hit[d] |= 1u << bd;
+          const int o = kBlkHead + 2 * G + d * WP + kBlkPad + min(x, W - 1);  // < 256
+          packed |= (uint32_t)o << (8 * bd);
+        }
+        reinterpret_cast<uint32_t*>(sp)[1 + d] = packed;
+      }
+      uint32_t mask = 0u;
+      for (int b0 = 0; b0 < nb[0]; ++b0)
+        for (int b1 = 0; b1 < nb[1]; ++b1)
+          for (int b2 = 0; b2 < nb[2]; ++b2)
+            if ((hit[0] >> b0) & (hit[1] >> b1) & (hit[2] >> b2) & 1u)
+              mask |= 1u << ((b0 * nb[1] + b1) * nb[2] + b2);
+      reinterpret_cast<uint32_t*>(sp)[0] = mask;
+      float* sv = sp + kBlkHead;
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        float2 v = make_float2(0.f, 0.f);
+        if (g < Gv) v = a.in[(int64_t)(t0 + g) * a.in_stride + p];
+        sv[2 * g] = v.x;
+        sv[2 * g + 1] = v.y;
+      }
+    }
+    __syncthreads();
+    // two entries per step: their headers (16-byte aligned) are read
+    // together, then entry q is summed before entry q + 1 (list order)
+    for (int q = 0; q < n; q += 2) {
+      const bool two = q + 1 < n;  // uniform
+      const uint4 h0 = *reinterpret_cast<const uint4*>(smem + q * PW);
+      const uint4 h1 = *reinterpret_cast<const uint4*>(smem + (two ? q + 1 : q) * PW);
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        if (k == 1 && !two) break;  // uniform
+        const uint4 h = k == 0 ? h0 : h1;
+        const float* sp = smem + (q + k) * PW;
+        const uint32_t mask = __builtin_amdgcn_readfirstlane(h.x);
+        bool any = false;
+#pragma unroll
+        for (int bi = 0; bi < kBlkPerWave; ++bi) any = any || (mask & bbit[bi]);
+        if (!any) continue;  // uniform
+        const uint32_t pk[3] = {(uint32_t)__builtin_amdgcn_readfirstlane(h.y),
+                                (uint32_t)__builtin_amdgcn_readfirstlane(h.z),
+                                (uint32_t)__builtin_amdgcn_readfirstlane(h.w)};
+        float2 v[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) v[g] = reinterpret_cast<const float2*>(sp + kBlkHead)[g];
+#pragma unroll
+        for (int bi = 0; bi < kBlkPerWave; ++bi) {
+          if (!(mask & bbit[bi])) continue;  // uniform
+          const int ox = (pk[0] >> (8 * bidx[bi][0])) & 255;
+          const int oy = (pk[1] >> (8 * bidx[bi][1])) & 255;
+          const int oz = (pk[2] >> (8 * bidx[bi][2])) & 255;
+          const float wx = sp[ox + lc[0]];
+          const float wy = sp[oy + lc[1]];
+          const float wz = sp[oz + lc[2]];
+          const float f = (wx * wy) * wz;
+#pragma unroll
+          for (int g = 0; g < G; ++g) {
+            acc[bi][g].x += f * v[g].x;
+            acc[bi][g].y += f * v[g].y;
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+
+  const int TC = td[0] * td[1] * td[2];
+#pragma unroll
+  for (int bi = 0; bi < kBlkPerWave; ++bi) {
+    if (!own[bi]) continue;
+    const int c0 = bidx[bi][0] * kBlkE + lc[0];
+    const int c1 = bidx[bi][1] * kBlkE + lc[1];
+    const int c2 = bidx[bi][2] * kBlkE + lc[2];
+    if (slot >= 0) {  // partial tile of a dense tile: summed by k_spread_reduce
+      const int r = (c0 * td[1] + c1) * td[2] + c2;
+#pragma unroll
+      for (int g = 0; g < G; ++g)
+        if (g < Gv) a.part[((int64_t)slot * a.ntrans + tb + t0 + g) * TC + r] = acc[bi][g];
+      continue;
+    }
+    const int64_t gi = ((int64_t)(org[0] + c0) * nf[1] + org[1] + c1) * nf[2] + org[2] + c2;
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+      if (g < Gv) a.grid[(int64_t)(tb + t0 + g) * a.grid_stride + gi] = acc[bi][g];
+  }
+}
+
 // Column-owner spreading for 3D tiles of 8 x 8 x 16 cells and 16 transforms
 // per workgroup: lane c of every wave owns tile column c (16 cells along
 // the contiguous axis), wave v owns transforms [4v, 4v + 4).  Per staged
@@ -446,9 +643,29 @@ __global__ __launch_bounds__(kThreads) void k_spread_reduce(SpreadArgs a) {
 // ---------------------------------------------------------------------------
 // interpolation (and the kernel-derivative gradient)
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ int wrap_coord(int x, int nfd, int wrap1) {
+  if (wrap1) {  // x inside [-nf, 2nf)
+    if (x < 0) x += nfd;
+    if (x >= nfd) x -= nfd;
+  } else {      // the region can exceed the grid several times over (tiny grids)
+    x %= nfd;
+    if (x < 0) x += nfd;
+  }
+  return x;
+}
+
+// Stage the tile + halo region of transforms [t0, t0 + G) into LDS: cell
+// (gg, l_0, .., l_{D-1}) at gg * RC + row-major l.  In 2D and 3D the region
+// is NR rows of RL cells along the last axis: a table after the staged
+// cells holds each row's wrapped grid offset, each thread owns one position
+// along the last axis (its wrapped grid coordinate computed once) and walks
+// the rows, kStageBatch loads in flight before their LDS stores.  The table
+// is built here and synchronised before the loads; the caller synchronises
+// after.
 template <int D, int W>
 __device__ __forceinline__ void stage_region(const InterpArgs& a, float2* lds, int tile, int t0,
                                              int G, int* rorg, int* R, int& RC) {
+  constexpr int kStageBatch = 24;
   int tc[3];
   tile_coords<D>(tile, a.g.nt, tc);
   RC = 1;
@@ -458,46 +675,69 @@ __device__ __forceinline__ void stage_region(const InterpArgs& a, float2* lds, i
     rorg[d] = tc[d] * a.g.tile[d] - W / 2;
     RC *= R[d];
   }
-  // kStageBatch loads per thread are in flight before their LDS stores; the
-  // cell index is split by multiply-high divisions (c < 2^16, so exact)
-  constexpr int kStageBatch = 24;
-  const int total = G * RC;
-  for (int c0 = threadIdx.x; c0 < total; c0 += kStageBatch * kThreads) {
-    float2 v[kStageBatch];
+  if constexpr (D == 1) {
+    const int nfd = a.g.nf[0];
+    for (int gg = 0; gg < G; ++gg) {
+      const float2* src = a.grid + (int64_t)(t0 + gg) * a.grid_stride;
+      float2* dst = lds + gg * RC;
+      for (int c0 = threadIdx.x; c0 < RC; c0 += kStageBatch * kThreads) {
+        float2 v[kStageBatch];
 #pragma unroll
-    for (int u = 0; u < kStageBatch; ++u) {
-      const int c = c0 + u * kThreads;
-      if (c < total) {
-        const int gg = G > 1 ? mdiv(c, a.rcmag) : 0;
-        int r = c - gg * RC;
-        int l[3];
-#pragma unroll
-        for (int d = D - 1; d >= 0; --d) {
-          const int q = d > 0 ? mdiv(r, a.rmag[d]) : 0;
-          l[d] = d > 0 ? r - q * R[d] : r;
-          r = q;
+        for (int u = 0; u < kStageBatch; ++u) {
+          const int c = c0 + u * kThreads;
+          if (c < RC) v[u] = src[wrap_coord(rorg[0] + c, nfd, a.wrap1)];
         }
-        int64_t gi = 0;
 #pragma unroll
-        for (int d = 0; d < D; ++d) {
-          const int nfd = a.g.nf[d];
-          int x = rorg[d] + l[d];
-          if (a.wrap1) {  // region inside [-nf, 2nf)
-            if (x < 0) x += nfd;
-            if (x >= nfd) x -= nfd;
-          } else {  // the region can exceed the grid several times over (tiny grids)
-            x %= nfd;
-            if (x < 0) x += nfd;
-          }
-          gi = gi * nfd + x;
+        for (int u = 0; u < kStageBatch; ++u) {
+          const int c = c0 + u * kThreads;
+          if (c < RC) dst[c] = v[u];
         }
-        v[u] = a.grid[(int64_t)(t0 + gg) * a.grid_stride + gi];
       }
     }
+  } else {
+    constexpr int DL = D - 1;
+    const int RL = R[DL];
+    const int NR = D == 2 ? R[0] : R[0] * R[1];
+    int* rowoff = reinterpret_cast<int*>(lds + G * RC);
+    for (int r = threadIdx.x; r < NR; r += kThreads) {
+      int off;
+      if constexpr (D == 2) {
+        off = wrap_coord(rorg[0] + r, a.g.nf[0], a.wrap1);
+      } else {
+        const int l0 = mdiv(r, a.rmag[1]);
+        const int l1 = r - l0 * R[1];
+        off = wrap_coord(rorg[0] + l0, a.g.nf[0], a.wrap1) * a.g.nf[1] +
+              wrap_coord(rorg[1] + l1, a.g.nf[1], a.wrap1);
+      }
+      rowoff[r] = off * a.g.nf[DL];
+    }
+    __syncthreads();
+    // L = min(RL, kThreads) lanes per row, P = kThreads / L rows per pass
+    const int L = a.stage_lanes, P = a.stage_rows;
+    const int rs = mdiv((int)threadIdx.x, a.lane_mag);
+    const int lz = (int)threadIdx.x - rs * L;
+    if (rs >= P) return;
+    for (int gg = 0; gg < G; ++gg) {
+      const float2* src = a.grid + (int64_t)(t0 + gg) * a.grid_stride;
+      float2* dst = lds + gg * RC;
+      for (int z = lz; z < RL; z += L) {
+        const int xz = wrap_coord(rorg[DL] + z, a.g.nf[DL], a.wrap1);
+        for (int r0 = rs; r0 < NR; r0 += kStageBatch * P) {
+          // each load waits for its table read: issuing all of them back to
+          // back measured slower (the loads of every wave arrive at once)
+          float2 v[kStageBatch];
 #pragma unroll
-    for (int u = 0; u < kStageBatch; ++u) {
-      const int c = c0 + u * kThreads;
-      if (c < total) lds[c] = v[u];
+          for (int u = 0; u < kStageBatch; ++u) {
+            const int r = r0 + u * P;
+            if (r < NR) v[u] = src[(uint32_t)(rowoff[r] + xz)];
+          }
+#pragma unroll
+          for (int u = 0; u < kStageBatch; ++u) {
+            const int r = r0 + u * P;
+            if (r < NR) dst[r * RL + z] = v[u];
+          }
+        }
+      }
     }
   }
 }
@@ -784,7 +1024,18 @@ static void set_region_magic(const hipnufft_plan_s* p, InterpArgs* a) {
     a->rmag[d] = magic_of(R);
     if (p->nf[o + d] < R) a->wrap1 = 0;
   }
-  a->rcmag = magic_of(tile_cells(p, p->w));
+  const int RL = p->tile[2] + p->w;
+  a->stage_lanes = std::min(RL, kThreads);
+  a->stage_rows = kThreads / a->stage_lanes;
+  a->lane_mag = magic_of(a->stage_lanes);
+}
+
+// LDS of an interpolation workgroup staging G transforms: the regions and,
+// in 2D / 3D, the table of row offsets after them
+static size_t region_lds_bytes(const hipnufft_plan_s* p, int G) {
+  const size_t cells = (size_t)tile_cells(p, p->w);
+  const size_t rows = p->ndim == 1 ? 0 : cells / (size_t)(p->tile[2] + p->w);
+  return sizeof(float2) * cells * (size_t)G + sizeof(int) * rows;
 }
 
 static int check_range(const hipnufft_plan_s* p, SetRange r) {
@@ -869,10 +1120,60 @@ int launch_spread_sorted(const hipnufft_plan_s* p, const float2* in, SetRange r,
     if (ncol * ((zl + kZ - 1) / kZ) > kThreads)
       return set_error(HIPNUFFT_ERR_ARG, "spread tile has more than 256 column segments of 4 cells");
   }
-  // column owners for 3D 8 x 8 x 16 tiles when asked for (spread_group >= 16
-  // and at least 8 transforms per set); otherwise the cell-owner kernel
-  if (p->ndim == 3 && p->tile[0] == 8 && p->tile[1] == 8 && p->tile[2] == kColsZ &&
-      r.ts >= 8 && p->spread_group >= 16) {
+  // kernel choice: column owners for 3D 8 x 8 x 16 tiles when asked for
+  // (spread_group >= 16 and at least 8 transforms per set), else block
+  // owners for 3D tiles of 4^3-cell blocks (edges 4, 8, 16), else the
+  // cell-owner kernel
+  const bool cols_ok = p->ndim == 3 && p->tile[0] == 8 && p->tile[1] == 8 &&
+                       p->tile[2] == kColsZ && r.ts >= 8;
+  bool blocks_ok = p->ndim == 3;
+  {
+    int nblk = 1;
+    for (int d = 0; d < 3 && blocks_ok; ++d) {
+      blocks_ok = p->tile[d] % kBlkE == 0 && p->tile[d] <= 4 * kBlkE &&
+                  p->nf[d] >= p->w + kBlkPad;
+      nblk *= p->tile[d] / kBlkE;
+    }
+    blocks_ok = blocks_ok && nblk % 4 == 0 && nblk <= 4 * kBlkPerWave;
+  }
+  int method = p->spread_method;
+  if (method == HIPNUFFT_SPREAD_AUTO)
+    method = (cols_ok && p->spread_group >= 16) ? HIPNUFFT_SPREAD_COLUMNS
+             : blocks_ok                        ? HIPNUFFT_SPREAD_BLOCKS
+                                                : HIPNUFFT_SPREAD_CELLS;
+  if (method == HIPNUFFT_SPREAD_COLUMNS && !cols_ok)
+    return set_error(HIPNUFFT_ERR_ARG,
+                     "column spreading needs 3D 8 x 8 x 16 tiles and >= 8 transforms per set");
+  if (method == HIPNUFFT_SPREAD_BLOCKS && !blocks_ok)
+    return set_error(HIPNUFFT_ERR_ARG,
+                     "block spreading needs 3D tiles of 4, 8 or 16 blocks of 4^3 cells, "
+                     "at most 4 along an axis");
+  if (method == HIPNUFFT_SPREAD_BLOCKS) {
+    int G = p->spread_group > 0 ? p->spread_group : 8;
+    G = G >= 8 ? 8 : G >= 4 ? 4 : G >= 2 ? 2 : 1;
+    while (G > 1 && G / 2 >= r.ts) G /= 2;
+    a.G = G;
+    const size_t lds =
+        sizeof(float) * (size_t)kBlkChunk * blk_record_floats(p->w, G);
+    if (nitems > 0) {
+      const dim3 grd((unsigned)nitems, (unsigned)((r.ts + G - 1) / G));
+      StageScope timer(p, kStageSpread);
+      switch (G) {
+        case 8: hipLaunchKernelGGL(k_spread_blocks<8>, grd, dim3(kThreads), lds, p->stream, a); break;
+        case 4: hipLaunchKernelGGL(k_spread_blocks<4>, grd, dim3(kThreads), lds, p->stream, a); break;
+        case 2: hipLaunchKernelGGL(k_spread_blocks<2>, grd, dim3(kThreads), lds, p->stream, a); break;
+        default: hipLaunchKernelGGL(k_spread_blocks<1>, grd, dim3(kThreads), lds, p->stream, a); break;
+      }
+      HN_LAUNCH_CHECK();
+    }
+    if (ndense > 0) {
+      const dim3 rg((unsigned)ndense, (unsigned)r.ts);
+      hipLaunchKernelGGL(k_spread_reduce<3>, rg, dim3(kThreads), 0, p->stream, a);
+      HN_LAUNCH_CHECK();
+    }
+    return 0;
+  }
+  if (method == HIPNUFFT_SPREAD_COLUMNS) {
     const size_t lds = sizeof(float) * (size_t)kColsChunk * (3 + 3 * p->w + 2 * 4 * kColsCW);
     if (nitems > 0) {
       const dim3 grd((unsigned)nitems, (unsigned)((r.ts + 4 * kColsCW - 1) / (4 * kColsCW)));
@@ -945,12 +1246,12 @@ int launch_interp(const hipnufft_plan_s* p, const float2* grid, int64_t grid_str
   a.scale = scale;
   a.mode = mode;
   set_region_magic(p, &a);
-  const size_t cell_bytes = sizeof(float2) * (size_t)tile_cells(p, p->w);
-  if (cell_bytes > kMaxLds) return set_error(HIPNUFFT_ERR_ARG, "interp tile too large for LDS");
+  if (region_lds_bytes(p, 1) > kMaxLds)
+    return set_error(HIPNUFFT_ERR_ARG, "interp tile too large for LDS");
   int G = std::min(p->interp_group, r.ts);
-  while (G > 1 && cell_bytes * G > kMaxLds) --G;
+  while (G > 1 && region_lds_bytes(p, G) > kMaxLds) --G;
   a.G = G;
-  const size_t lds = cell_bytes * G;
+  const size_t lds = region_lds_bytes(p, G);
   const dim3 grd((unsigned)nsub, (unsigned)((r.ts + G - 1) / G));
   StageScope timer(p, kStageInterp);
   HN_DIM_SWITCH(p->ndim, HN_WIDTH_SWITCH(p->w, {
@@ -983,16 +1284,17 @@ int launch_grad_traj(const hipnufft_plan_s* p, const float2* grid, int64_t grid_
   set_region_magic(p, &a);
   const int o = 3 - p->ndim;
   for (int d = 0; d < p->ndim; ++d) a.dscale[d] = (float)((double)p->nf[o + d] / (2.0 * M_PI));
-  const size_t cell_bytes = sizeof(float2) * (size_t)tile_cells(p, p->w);
   // each workgroup handles the transforms of its set that fit LDS; the
   // launches accumulate into gtraj in stream order
-  const int G = (int)(kMaxLds / cell_bytes);
-  if (G < 1) return set_error(HIPNUFFT_ERR_ARG, "interp tile too large for LDS");
+  if (region_lds_bytes(p, 1) > kMaxLds)
+    return set_error(HIPNUFFT_ERR_ARG, "interp tile too large for LDS");
+  int G = 1;
+  while (region_lds_bytes(p, G + 1) <= kMaxLds) ++G;
   for (int tg0 = 0; tg0 < r.ts; tg0 += G) {
     const int Tg = std::min(G, r.ts - tg0);
     a.tg0 = tg0;
     a.G = Tg;
-    const size_t lds = cell_bytes * Tg;
+    const size_t lds = region_lds_bytes(p, Tg);
     StageScope timer(p, kStageGradTraj);
     HN_DIM_SWITCH(p->ndim, HN_WIDTH_SWITCH(p->w, {
       hipLaunchKernelGGL((k_grad_traj<DD, WW>), dim3((unsigned)nsub), dim3(kThreads), lds,

@@ -46,6 +46,7 @@ class Opts(ctypes.Structure):
         ("max_subproblem", ctypes.c_int),
         ("deterministic", ctypes.c_int),
         ("spread_max", ctypes.c_int),
+        ("spread_method", ctypes.c_int),
     ]
 
 
@@ -137,6 +138,18 @@ def current_stream() -> int:
     return torch.cuda.current_stream().cuda_stream
 
 
+SPREAD_METHODS = {"auto": 0, "cells": 1, "blocks": 2, "columns": 3}
+
+
+def spread_method_id(m):
+    """HIPNUFFT_SPREAD_* value of a spreading-kernel name or number."""
+    if isinstance(m, str):
+        if m not in SPREAD_METHODS:
+            raise ValueError(f"spread_method must be one of {sorted(SPREAD_METHODS)}")
+        return SPREAD_METHODS[m]
+    return int(m)
+
+
 class RawPlan:
     """Owns one ``hipnufft_plan`` (both transform directions).
 
@@ -147,7 +160,8 @@ class RawPlan:
 
     def __init__(self, shape, n_trans=1, eps=1e-6, isign=-1, upsampfac=2.0,
                  spreadinterponly=False, tile=None, spread_group=0,
-                 interp_group=0, max_subproblem=0, spread_max=0, stream=None):
+                 interp_group=0, max_subproblem=0, spread_max=0, spread_method=0,
+                 stream=None):
         L = lib()
         self.shape = tuple(int(s) for s in shape)
         self.ndim = len(self.shape)
@@ -165,6 +179,7 @@ class RawPlan:
         o.interp_group = int(interp_group)
         o.max_subproblem = int(max_subproblem)
         o.spread_max = int(spread_max)
+        o.spread_method = spread_method_id(spread_method)
         self.opts = o
         nm = (_I64 * self.ndim)(*self.shape)
         h = _P()

@@ -6,13 +6,15 @@ plan options and keeps the fastest, by coordinate search:
 
 1. tile shape (which sets both the bin sort and the spread / interp work
    granularity);
-2. transforms accumulated per spreading workgroup (``spread_group``);
-3. transforms gathered per interpolation workgroup (``interp_group``);
-4. dense-bin splitting (``max_subproblem``, ``spread_max``).
+2. in 3D, the spreading kernel (``spread_method``: cell or block owners);
+3. transforms accumulated per spreading workgroup (``spread_group``);
+4. transforms gathered per interpolation workgroup (``interp_group``);
+5. dense-bin splitting (``max_subproblem``, ``spread_max``).
 
 Only candidates the kernels accept are tried: a spreading tile needs at most
 256 (column, 4-cell segment) pairs, an interpolation region (tile + kernel
-width halo) at most 64 KiB of LDS.  Results are cached in-process by a
+width halo, with a 4-byte offset per row along the last axis) at most 64 KiB
+of LDS.  Results are cached in-process by a
 fingerprint of the trajectory (size, shape, coils, a strided subsample of the
 coordinates), and optionally in a JSON file (``HIPNUFFT_TUNE_CACHE``).
 
@@ -50,7 +52,9 @@ def tile_ok(tile, w, interp_group=1):
     zl = tile[-1]
     if (cells // zl) * -(-zl // 4) > 256:
         return False
-    return int(np.prod([t + w for t in tile])) * 8 * max(interp_group, 1) <= 65536
+    region = [t + w for t in tile]
+    rows = int(np.prod(region[:-1])) if len(tile) > 1 else 0   # row-offset table
+    return int(np.prod(region)) * 8 * max(interp_group, 1) + 4 * rows <= 65536
 
 
 def tile_candidates(ndim, w):
@@ -124,16 +128,21 @@ def autotune(coords, shape, n_trans=1, eps=1e-6, upsampfac=2.0, reps=3, budget_s
 
     def timed(opts):
         full = dict(upsampfac=upsampfac, **opts)
-        ms = _time_opts(coords, shape, n_trans, eps, full, reps, x, y)
+        try:
+            ms = _time_opts(coords, shape, n_trans, eps, full, reps, x, y)
+        except ValueError:  # a combination the kernels reject
+            ms = float("inf")
         tried.append((dict(opts), ms))
         if verbose:
             print(f"[autotune] {opts} -> {ms:.3f} ms", flush=True)
         return ms
 
-    best = dict(tile=None, spread_group=0, interp_group=0, max_subproblem=0, spread_max=0)
+    best = dict(tile=None, spread_group=0, interp_group=0, max_subproblem=0, spread_max=0,
+                spread_method=0)
     best_ms = timed(best)
     stages = [
         ("tile", tile_candidates(ndim, w)),
+        ("spread_method", [1, 2] if ndim == 3 else []),
         ("spread_group", [4, 8, 16]),
         ("interp_group", [1, 2, 4] if ndim < 3 else [1]),
         ("max_subproblem", [512, 1024, 2048]),

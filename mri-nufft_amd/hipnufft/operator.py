@@ -74,9 +74,10 @@ class MRIHipNUFFT(FourierOperatorBase, _ToggleGradPlanMixin):
     smaps_cached; verbose; squeeze_dims (default False); n_trans (None: all
     B*C transforms per launch within a 24 GiB workspace budget);
     async_transfer; and kwargs eps (1e-6), upsampfac (2.0), gpu_device_id,
-    tile, spread_group, interp_group, max_subproblem, spread_max, and
-    autotune (time candidate plan options on this trajectory and keep the
-    fastest; explicit options win).
+    tile, spread_group, interp_group, max_subproblem, spread_max,
+    spread_method ("auto", "cells", "blocks", "columns"), and autotune (time
+    candidate plan options on this trajectory and keep the fastest; explicit
+    options win).
     """
 
     backend = "hipnufft"
@@ -97,6 +98,11 @@ class MRIHipNUFFT(FourierOperatorBase, _ToggleGradPlanMixin):
         self.async_transfer = async_transfer
         self.verbose = verbose
         self._eps = float(kwargs.pop("eps", 1e-6))
+        # options given by the caller (not None / 0 / "auto") are kept over
+        # the autotuner's choice
+        explicit = {k for k in ("tile", "spread_group", "interp_group", "max_subproblem",
+                                "spread_max", "spread_method")
+                    if kwargs.get(k) not in (None, 0, "auto")}
         self._plan_opts = dict(
             upsampfac=float(kwargs.pop("upsampfac", 2.0)),
             spreadinterponly=bool(kwargs.pop("spreadinterponly", False)),
@@ -105,9 +111,8 @@ class MRIHipNUFFT(FourierOperatorBase, _ToggleGradPlanMixin):
             interp_group=int(kwargs.pop("interp_group", 0)),
             max_subproblem=int(kwargs.pop("max_subproblem", 0)),
             spread_max=int(kwargs.pop("spread_max", 0)),
+            spread_method=kwargs.pop("spread_method", 0),
         )
-        explicit = {k for k in ("tile", "spread_group", "interp_group", "max_subproblem",
-                                "spread_max") if kwargs.get(k) not in (None, 0)}
         do_tune = bool(kwargs.pop("autotune", False))
         for k in ("gpu_method", "gpu_sort", "gpu_kerevalmeth", "gpu_spreadinterponly"):
             kwargs.pop(k, None)

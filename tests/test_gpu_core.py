@@ -157,30 +157,77 @@ def test_dot_fp64():
     np.testing.assert_array_equal(r, r3)
 
 
-@pytest.mark.parametrize("T", [8, 11, 16, 19])
-def test_spread_column_kernel_equals_cell_kernel(T):
-    """3D 8x8x16 tiles: the column-owner spreading (spread_group=16, >= 8
-    transforms) and the cell-owner kernel (spread_group=8, scalar z-segment
-    tests) sum the same list entries in the same order with the same weights:
-    bit-identical grids.  Includes split dense tiles (spread_max) and a
-    partially filled last wave."""
+@pytest.mark.parametrize("T", [1, 3, 8, 11, 16, 19])
+def test_spread_kernels_bit_identical(T):
+    """3D 8x8x16 tiles: the cell-owner, block-owner and (>= 8 transforms)
+    column-owner spreading kernels sum the same list entries in the same
+    order with the same weights: bit-identical grids.  Includes split dense
+    tiles (spread_max) and a partially filled last transform group."""
     from hipnufft import _capi
 
     traj = np.concatenate([rand_traj(3000, 3, 41), rand_traj(2500, 3, 42) * 0.06])
     shape = (24, 20, 32)
     om = torch.from_numpy((traj * np.float32(2 * np.pi)).astype(np.float32)).to(DEV)
     y = torch.from_numpy(rand_cplx((T, traj.shape[0]), seed=43)).to(DEV)
-    outs = []
-    for sg in (8, 16):
-        p = _capi.RawPlan(shape, n_trans=T, spread_group=sg, spread_max=300)
+    methods = ["cells", "blocks"] + (["columns"] if T >= 8 else [])
+    outs = {}
+    for m in methods:
+        p = _capi.RawPlan(shape, n_trans=T, spread_group=16 if m == "columns" else 8,
+                          spread_max=300, spread_method=m)
         p.setpts(om)
         assert p.info()["tile"] == (8, 8, 16)
         x = torch.empty((T, *shape), dtype=torch.complex64, device=DEV)
         p.execute_adjoint(y, x)
-        outs.append(x)
+        outs[m] = x
     torch.cuda.synchronize()
-    assert torch.equal(outs[0], outs[1])
+    for m in methods[1:]:
+        assert torch.equal(outs["cells"], outs[m]), m
     ref = nufft.Plan(om.cpu().numpy(), shape, eps=1e-6)
     for t in (0, T - 1):
         exp = ref.execute_adjoint(y[t].cpu().numpy().astype(np.complex128))
-        assert rel_l2(outs[1][t].cpu().numpy(), exp) < 1e-5
+        assert rel_l2(outs["blocks"][t].cpu().numpy(), exp) < 1e-5
+
+
+@pytest.mark.parametrize("tile,shape,eps", [
+    ((8, 8, 8), (16, 12, 10), 1e-6),     # fine grid 32 x 24 x 20: partial edge tiles
+    ((4, 8, 16), (20, 16, 24), 1e-6),
+    ((4, 4, 16), (12, 14, 18), 1e-3),    # w = 4, a last-axis tile past the grid end
+    ((8, 4, 8), (6, 6, 6), 1e-2),        # w = 3, nf = 12: one block per wave, wraps often
+])
+def test_spread_blocks_other_tiles(tile, shape, eps):
+    """Block-owner spreading on other tile shapes (1 or 2 blocks per wave,
+    tiles cut by the grid end, small grids where footprints wrap): bit-equal
+    to the cell-owner kernel and within eps of the FP64 oracle."""
+    from hipnufft import _capi
+
+    traj = np.concatenate([rand_traj(1500, 3, 44), rand_traj(800, 3, 45) * 0.1])
+    om = torch.from_numpy((traj * np.float32(2 * np.pi)).astype(np.float32)).to(DEV)
+    T = 5
+    y = torch.from_numpy(rand_cplx((T, traj.shape[0]), seed=46)).to(DEV)
+    outs = {}
+    for m in ("cells", "blocks"):
+        p = _capi.RawPlan(shape, n_trans=T, eps=eps, tile=tile, spread_max=200, spread_method=m)
+        p.setpts(om)
+        assert p.info()["tile"] == tile
+        x = torch.empty((T, *shape), dtype=torch.complex64, device=DEV)
+        p.execute_adjoint(y, x)
+        outs[m] = x
+    torch.cuda.synchronize()
+    assert torch.equal(outs["cells"], outs["blocks"])
+    ref = nufft.Plan(om.cpu().numpy(), shape, eps=eps)
+    exp = ref.execute_adjoint(y[T - 1].cpu().numpy().astype(np.complex128))
+    assert rel_l2(outs["blocks"][T - 1].cpu().numpy(), exp) < 10 * eps
+
+
+def test_spread_method_rejected_where_unsupported():
+    from hipnufft import _capi
+
+    om = torch.from_numpy((rand_traj(500, 2, 47) * np.float32(2 * np.pi)).astype(np.float32)).to(DEV)
+    y = torch.from_numpy(rand_cplx((1, 500), seed=48)).to(DEV)
+    p = _capi.RawPlan((32, 32), n_trans=1, spread_method="blocks")
+    p.setpts(om)
+    x = torch.empty((1, 32, 32), dtype=torch.complex64, device=DEV)
+    with pytest.raises(ValueError, match="block spreading"):
+        p.execute_adjoint(y, x)
+    with pytest.raises(ValueError):
+        _capi.RawPlan((32, 32), n_trans=1, spread_method="fastest")
