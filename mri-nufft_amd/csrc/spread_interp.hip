@@ -282,9 +282,10 @@ __global__ __launch_bounds__(kThreads) void k_spread(SpreadArgs a) {
 // blocks v, v + 4, ...).  When an entry is staged, its thread records which
 // blocks of the tile the footprint touches (a bit mask) and, per axis and
 // block along it, where the lane weights of that block start in the entry's
-// weight rows (padded with 3 zeros on each side), so a wave skips untouched
-// blocks with one bit test and a touched block costs three weight reads
-// and the products per lane.  A cell outside the footprint adds f = 0 times
+// weight rows (padded with 3 zeros on each side).  A wave selects the
+// entries that touch its blocks with one ballot per 64 entries and walks
+// only those; a touched block costs three weight reads and the products per
+// lane.  A cell outside the footprint adds f = 0 times
 // the strength: its sum is unchanged, and every cell's sum is formed in
 // list order with the products (w0 * w1) * w2 of the cell-owner kernel:
 // the same bits.
@@ -407,25 +408,25 @@ __global__ __launch_bounds__(kThreads) void k_spread_blocks(SpreadArgs a) {
       }
     }
     __syncthreads();
-    // two entries per step: their headers (16-byte aligned) are read
-    // together, then entry q is summed before entry q + 1 (list order)
-    for (int q = 0; q < n; q += 2) {
-      const bool two = q + 1 < n;  // uniform
-      const uint4 h0 = *reinterpret_cast<const uint4*>(smem + q * PW);
-      const uint4 h1 = *reinterpret_cast<const uint4*>(smem + (two ? q + 1 : q) * PW);
+    // per 64 staged entries: lane i reads the header of entry base + i, a
+    // ballot marks the entries that touch one of the wave's blocks, and the
+    // wave walks only those, lowest first (list order kept)
+    uint32_t wbits = 0u;
 #pragma unroll
-      for (int k = 0; k < 2; ++k) {
-        if (k == 1 && !two) break;  // uniform
-        const uint4 h = k == 0 ? h0 : h1;
-        const float* sp = smem + (q + k) * PW;
-        const uint32_t mask = __builtin_amdgcn_readfirstlane(h.x);
-        bool any = false;
-#pragma unroll
-        for (int bi = 0; bi < kBlkPerWave; ++bi) any = any || (mask & bbit[bi]);
-        if (!any) continue;  // uniform
-        const uint32_t pk[3] = {(uint32_t)__builtin_amdgcn_readfirstlane(h.y),
-                                (uint32_t)__builtin_amdgcn_readfirstlane(h.z),
-                                (uint32_t)__builtin_amdgcn_readfirstlane(h.w)};
+    for (int bi = 0; bi < kBlkPerWave; ++bi) wbits |= bbit[bi];
+    for (int base = 0; base < n; base += 64) {
+      const int qi = base + lane;
+      uint4 h = make_uint4(0u, 0u, 0u, 0u);
+      if (qi < n) h = *reinterpret_cast<const uint4*>(smem + qi * PW);
+      uint64_t todo = __ballot((h.x & wbits) != 0u);
+      while (todo) {  // uniform
+        const int i = __builtin_ctzll(todo);
+        todo &= todo - 1;
+        const float* sp = smem + (base + i) * PW;
+        const uint32_t mask = (uint32_t)__builtin_amdgcn_readlane((int)h.x, i);
+        const uint32_t pk[3] = {(uint32_t)__builtin_amdgcn_readlane((int)h.y, i),
+                                (uint32_t)__builtin_amdgcn_readlane((int)h.z, i),
+                                (uint32_t)__builtin_amdgcn_readlane((int)h.w, i)};
         float2 v[G];
 #pragma unroll
         for (int g = 0; g < G; ++g) v[g] = reinterpret_cast<const float2*>(sp + kBlkHead)[g];
