@@ -22,7 +22,6 @@
 namespace hn {
 namespace {
 
-constexpr int kEPT = kMaxElemsFFT / kThreads;  // complex elements per thread (16)
 
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
@@ -120,10 +119,10 @@ __device__ __forceinline__ void dftR(float2* v) {
 // One Stockham radix-R pass over L interleaved lines of length n held in LDS
 // at buf[pos * LP + line].  Every thread first reads all its butterflies into
 // registers, then (after a barrier) writes the results: a single LDS buffer.
-template <int R, int S>
+template <int R, int S, int EPT>
 __device__ __forceinline__ void stockham_pass(float2* buf, int n, int lgL, int LP, int Ns,
                                               uint32_t ns_magic, const float2* tw) {
-  constexpr int MAXB = (kEPT + R - 1) / R;
+  constexpr int MAXB = (EPT + R - 1) / R;
   const int L = 1 << lgL;
   // uniform (scalar) quantities; every LDS address below is one 24-bit
   // multiply per butterfly plus uniform strides per radix leg
@@ -173,18 +172,18 @@ __device__ __forceinline__ void stockham_pass(float2* buf, int n, int lgL, int L
 
 // All passes of the FFT of the L lines in buf; twl is the twiddle table
 // staged in LDS by stage_twiddles (visible after the caller's barrier).
-template <int S>
+template <int S, int EPT>
 __device__ void fft_lines(float2* buf, const float2* twl, const FFTAxis& ax, int lgL, int LP) {
   int Ns = 1;
   for (int p = 0; p < ax.nrad; ++p) {
     const int R = ax.rad[p];
     const uint32_t mg = ax.ns_magic[p];
     switch (R) {
-      case 2: stockham_pass<2, S>(buf, ax.n, lgL, LP, Ns, mg, twl); break;
-      case 3: stockham_pass<3, S>(buf, ax.n, lgL, LP, Ns, mg, twl); break;
-      case 4: stockham_pass<4, S>(buf, ax.n, lgL, LP, Ns, mg, twl); break;
-      case 5: stockham_pass<5, S>(buf, ax.n, lgL, LP, Ns, mg, twl); break;
-      default: stockham_pass<8, S>(buf, ax.n, lgL, LP, Ns, mg, twl); break;
+      case 2: stockham_pass<2, S, EPT>(buf, ax.n, lgL, LP, Ns, mg, twl); break;
+      case 3: stockham_pass<3, S, EPT>(buf, ax.n, lgL, LP, Ns, mg, twl); break;
+      case 4: stockham_pass<4, S, EPT>(buf, ax.n, lgL, LP, Ns, mg, twl); break;
+      case 5: stockham_pass<5, S, EPT>(buf, ax.n, lgL, LP, Ns, mg, twl); break;
+      default: stockham_pass<8, S, EPT>(buf, ax.n, lgL, LP, Ns, mg, twl); break;
     }
     Ns *= R;
   }
@@ -225,7 +224,7 @@ struct RowsArgs {
 };
 
 // image rows -> deconvolve * scale (* smap) -> pad -> FFT -> grid rows
-template <int S>
+template <int S, int EPT>
 __global__ __launch_bounds__(kThreads) void k_rows_fwd(RowsArgs a) {
   extern __shared__ float2 buf[];
   const int L = 1 << a.lgL, LP = L + 1;
@@ -238,7 +237,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_fwd(RowsArgs a) {
   const int64_t N = (int64_t)rows * a.M2;
   const int n = a.nf2;
   // line-major thread mapping: thread (l, x0) handles positions x0 + j * TPL
-  // of row r0 + l (n <= TPL * kEPT), so it keeps one row base and one scale
+  // of row r0 + l (n <= TPL * EPT), so it keeps one row base and one scale
   const int TPL = kThreads >> a.lgL;
   const int l = threadIdx.x / TPL, x0 = threadIdx.x - l * TPL;
   const int r = r0 + l;
@@ -251,9 +250,9 @@ __global__ __launch_bounds__(kThreads) void k_rows_fwd(RowsArgs a) {
   const int M2 = a.M2, h2 = M2 / 2;
 
   // loads first (overlapping), then the arithmetic and LDS stores
-  float2 v[kEPT], sv[kEPT];
+  float2 v[EPT], sv[EPT];
 #pragma unroll
-  for (int j = 0; j < kEPT; ++j) {
+  for (int j = 0; j < EPT; ++j) {
     const int x = x0 + j * TPL;
     v[j] = make_float2(0.f, 0.f);
     sv[j] = make_float2(1.f, 0.f);
@@ -263,7 +262,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_fwd(RowsArgs a) {
     }
   }
 #pragma unroll
-  for (int j = 0; j < kEPT; ++j) {
+  for (int j = 0; j < EPT; ++j) {
     const int x = x0 + j * TPL;
     if (x < M2) {
       float2 s2 = sv[j];
@@ -274,17 +273,17 @@ __global__ __launch_bounds__(kThreads) void k_rows_fwd(RowsArgs a) {
   }
   const int gap0 = M2 - h2, gap = n - M2;
 #pragma unroll
-  for (int j = 0; j < kEPT; ++j) {
+  for (int j = 0; j < EPT; ++j) {
     const int e = x0 + j * TPL;
     if (e < gap) buf[(gap0 + e) * LP + l] = make_float2(0.f, 0.f);
   }
   __syncthreads();
-  fft_lines<S>(buf, twl, a.ax, a.lgL, LP);
+  fft_lines<S, EPT>(buf, twl, a.ax, a.lgL, LP);
   if (!rok) return;
   float2* g = a.grid + (int64_t)t * a.gstride +
               ((int64_t)kept_pos(i0, a.M0, a.nf0) * a.nf1 + kept_pos(i1, a.M1, a.nf1)) * n;
 #pragma unroll
-  for (int j = 0; j < kEPT; ++j) {
+  for (int j = 0; j < EPT; ++j) {
     const int x = x0 + j * TPL;
     if (x < n) g[x] = buf[x * LP + l];
   }
@@ -292,7 +291,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_fwd(RowsArgs a) {
 
 // grid rows -> FFT -> crop -> deconvolve * scale (* conj smap, summed over the
 // chunk's coils of one batch) -> image rows
-template <int S>
+template <int S, int EPT>
 __global__ __launch_bounds__(kThreads) void k_rows_adj(RowsArgs a) {
   extern __shared__ float2 buf[];
   const int L = 1 << a.lgL, LP = L + 1;
@@ -316,7 +315,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_adj(RowsArgs a) {
   const bool add_existing = a.smaps && (tb % a.C != 0);
   // line-major thread mapping: thread (l, x0) handles positions x0 + j * TPL
   // of line l, so it keeps one row base across the coil loop
-  const int TPL = kThreads >> a.lgL;  // threads per line (n <= TPL * kEPT)
+  const int TPL = kThreads >> a.lgL;  // threads per line (n <= TPL * EPT)
   const int l = threadIdx.x / TPL, x0 = threadIdx.x - l * TPL;
   const int r = r0 + l;
   const bool rok = r < rows;
@@ -332,14 +331,14 @@ __global__ __launch_bounds__(kThreads) void k_rows_adj(RowsArgs a) {
   for (int tt = tb; tt < te; ++tt) {
     const float2* g = a.grid + (int64_t)(tt - a.t0) * a.gstride + grow;
     {
-      float2 v[kEPT];
+      float2 v[EPT];
 #pragma unroll
-      for (int j = 0; j < kEPT; ++j) {
+      for (int j = 0; j < EPT; ++j) {
         const int x = x0 + j * TPL;
         v[j] = (rok && x < n) ? g[x] : make_float2(0.f, 0.f);
       }
 #pragma unroll
-      for (int j = 0; j < kEPT; ++j) {
+      for (int j = 0; j < EPT; ++j) {
         const int x = x0 + j * TPL;
         if (x < n) buf[x * LP + l] = v[j];
       }
@@ -350,11 +349,11 @@ __global__ __launch_bounds__(kThreads) void k_rows_adj(RowsArgs a) {
     // coil loop into registers
     int lg = a.lgL, lp = LP;
     asm volatile("" : "+s"(lg), "+s"(lp));
-    fft_lines<S>(buf, twl, a.ax, lg, lp);
+    fft_lines<S, EPT>(buf, twl, a.ax, lg, lp);
     const float2* sm =
         a.smaps ? a.smaps + (int64_t)(a.smaps_all ? tt : tt % a.C) * N + (int64_t)r * a.M2 : nullptr;
 #pragma unroll
-    for (int j = 0; j < kEPT; ++j) {
+    for (int j = 0; j < EPT; ++j) {
       const int x = x0 + j * TPL;
       if (x < a.M2) {
         float2 v = make_float2(0.f, 0.f);
@@ -402,7 +401,7 @@ struct StridedArgs {
 // In-place FFT of L adjacent lines along a strided axis.  CONV: forward FFT,
 // multiply by the spectrum K, inverse FFT, all in LDS (the middle of the
 // Toeplitz convolution, one grid round trip instead of three).
-template <int S, bool CONV>
+template <int S, bool CONV, int EPT>
 __global__ __launch_bounds__(kThreads) void k_fft_strided(StridedArgs a) {
   extern __shared__ float2 buf[];
   const int L = 1 << a.lgL, LP = L + 1;
@@ -425,40 +424,40 @@ __global__ __launch_bounds__(kThreads) void k_fft_strided(StridedArgs a) {
   const int hin = a.in_M ? a.in_M / 2 : 0;  // kept position of e: e - h (+ n if e < h)
   const int gofs = (e0 - hin) * ls, lofs = (e0 - hin) * LP + l;
   // all of a thread's loads are issued before the first LDS store, so they
-  // overlap instead of paying one memory latency each (n * L <= kEPT * kThreads)
-  float2 v[kEPT];
+  // overlap instead of paying one memory latency each (n * L <= EPT * kThreads)
+  float2 v[EPT];
 #pragma unroll
-  for (int i = 0; i < kEPT; ++i) {
+  for (int i = 0; i < EPT; ++i) {
     const int e = e0 + i * E;
     v[i] = make_float2(0.f, 0.f);
     if (e < nin && lok) v[i] = base[gofs + i * (E * ls) + (e < hin ? nls : 0)];
   }
 #pragma unroll
-  for (int i = 0; i < kEPT; ++i) {
+  for (int i = 0; i < EPT; ++i) {
     const int e = e0 + i * E;
     if (e < nin) buf[lofs + i * (E * LP) + (e < hin ? nLP : 0)] = v[i];
   }
   if (a.in_M) {  // zero the positions between the kept halves
     const int gap0 = a.in_M - hin, gap = n - a.in_M;
 #pragma unroll
-    for (int i = 0; i < kEPT; ++i) {
+    for (int i = 0; i < EPT; ++i) {
       const int e = e0 + i * E;
       if (e < gap) buf[(gap0 + e) * LP + l] = make_float2(0.f, 0.f);
     }
   }
   __syncthreads();
-  fft_lines<S>(buf, twl, a.ax, a.lgL, LP);
+  fft_lines<S, EPT>(buf, twl, a.ax, a.lgL, LP);
   if constexpr (CONV) {
     const float2* kb = a.K + (int64_t)opos * a.ostride + x0 + l;
     const int kofs = e0 * ls;
 #pragma unroll
-    for (int i = 0; i < kEPT; ++i) {
+    for (int i = 0; i < EPT; ++i) {
       const int e = e0 + i * E;
       v[i] = make_float2(0.f, 0.f);
       if (e < n && lok) v[i] = kb[kofs + i * (E * ls)];
     }
 #pragma unroll
-    for (int i = 0; i < kEPT; ++i) {
+    for (int i = 0; i < EPT; ++i) {
       const int e = e0 + i * E;
       if (e < n) {
         const int q = e * LP + l;
@@ -466,14 +465,14 @@ __global__ __launch_bounds__(kThreads) void k_fft_strided(StridedArgs a) {
       }
     }
     __syncthreads();
-    fft_lines<-S>(buf, twl, a.ax, a.lgL, LP);
+    fft_lines<-S, EPT>(buf, twl, a.ax, a.lgL, LP);
   }
   const int nout = a.out_M ? a.out_M : n;
   const int hout = a.out_M ? a.out_M / 2 : 0;
   const int gofs2 = (e0 - hout) * ls, lofs2 = (e0 - hout) * LP + l;
   if (lok) {
 #pragma unroll
-    for (int i = 0; i < kEPT; ++i) {
+    for (int i = 0; i < EPT; ++i) {
       const int e = e0 + i * E;
       if (e < nout)
         base[gofs2 + i * (E * ls) + (e < hout ? nls : 0)] =
@@ -489,6 +488,21 @@ int lg2(int x) {
 }
 
 size_t fft_lds(const FFTAxis& ax) { return sizeof(float2) * (size_t)ax.n * (ax.L + 2); }
+
+// complex elements per thread for the L lines of n elements of an axis:
+// the smallest of 8, 12, 16 with n * L <= EPT * kThreads (fewer registers,
+// more resident waves for the shorter lines)
+int ept_of(const FFTAxis& ax) {
+  const int need = (ax.n * ax.L + kThreads - 1) / kThreads;
+  return need <= 8 ? 8 : need <= 12 ? 12 : 16;
+}
+
+#define HN_EPT_SWITCH(ept, ...)             \
+  switch (ept) {                            \
+    case 8: { constexpr int EE = 8; __VA_ARGS__ } break;   \
+    case 12: { constexpr int EE = 12; __VA_ARGS__ } break; \
+    default: { constexpr int EE = 16; __VA_ARGS__ } break; \
+  }
 
 }  // namespace
 
@@ -525,16 +539,19 @@ int strided(const hipnufft_plan_s* p, int axis, int T, int in_M, int out_M, int 
   a.K = K;
   const dim3 grd((unsigned)((a.nx + ax.L - 1) / ax.L), (unsigned)a.nouter, (unsigned)T);
   StageScope timer(p, stage);
-  if (K) {
-    if (sign < 0)
-      hipLaunchKernelGGL((k_fft_strided<-1, true>), grd, dim3(kThreads), fft_lds(ax), p->stream, a);
-    else
-      hipLaunchKernelGGL((k_fft_strided<1, true>), grd, dim3(kThreads), fft_lds(ax), p->stream, a);
-  } else if (sign < 0) {
-    hipLaunchKernelGGL((k_fft_strided<-1, false>), grd, dim3(kThreads), fft_lds(ax), p->stream, a);
-  } else {
-    hipLaunchKernelGGL((k_fft_strided<1, false>), grd, dim3(kThreads), fft_lds(ax), p->stream, a);
-  }
+  const size_t lds = fft_lds(ax);
+  HN_EPT_SWITCH(ept_of(ax), {
+    if (K) {
+      if (sign < 0)
+        hipLaunchKernelGGL((k_fft_strided<-1, true, EE>), grd, dim3(kThreads), lds, p->stream, a);
+      else
+        hipLaunchKernelGGL((k_fft_strided<1, true, EE>), grd, dim3(kThreads), lds, p->stream, a);
+    } else if (sign < 0) {
+      hipLaunchKernelGGL((k_fft_strided<-1, false, EE>), grd, dim3(kThreads), lds, p->stream, a);
+    } else {
+      hipLaunchKernelGGL((k_fft_strided<1, false, EE>), grd, dim3(kThreads), lds, p->stream, a);
+    }
+  });
   HN_LAUNCH_CHECK();
   return 0;
 }
@@ -601,10 +618,12 @@ int rows_fwd(const hipnufft_plan_s* p, const float2* img, const float2* smaps, i
   const size_t lds = fft_lds(p->fft[2]);
   {
     StageScope timer(p, kStageRowsFwd);
-    if (sign < 0)
-      hipLaunchKernelGGL(k_rows_fwd<-1>, grd, dim3(kThreads), lds, p->stream, a);
-    else
-      hipLaunchKernelGGL(k_rows_fwd<1>, grd, dim3(kThreads), lds, p->stream, a);
+    HN_EPT_SWITCH(ept_of(p->fft[2]), {
+      if (sign < 0)
+        hipLaunchKernelGGL((k_rows_fwd<-1, EE>), grd, dim3(kThreads), lds, p->stream, a);
+      else
+        hipLaunchKernelGGL((k_rows_fwd<1, EE>), grd, dim3(kThreads), lds, p->stream, a);
+    });
     HN_LAUNCH_CHECK();
   }
   return 0;
@@ -626,10 +645,12 @@ int rows_adj(const hipnufft_plan_s* p, float2* img, const float2* smaps, int C, 
   const dim3 grd((unsigned)((rows + L - 1) / L), (unsigned)nimg);
   const size_t lds = rows_adj_lds(p->fft[2], a.M2);
   StageScope timer(p, kStageRowsAdj);
-  if (sign < 0)
-    hipLaunchKernelGGL(k_rows_adj<-1>, grd, dim3(kThreads), lds, p->stream, a);
-  else
-    hipLaunchKernelGGL(k_rows_adj<1>, grd, dim3(kThreads), lds, p->stream, a);
+  HN_EPT_SWITCH(ept_of(p->fft[2]), {
+    if (sign < 0)
+      hipLaunchKernelGGL((k_rows_adj<-1, EE>), grd, dim3(kThreads), lds, p->stream, a);
+    else
+      hipLaunchKernelGGL((k_rows_adj<1, EE>), grd, dim3(kThreads), lds, p->stream, a);
+  });
   HN_LAUNCH_CHECK();
   return 0;
 }
