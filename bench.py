@@ -203,9 +203,10 @@ def main():
     ap.add_argument("--autotune", action="store_true",
                     help="choose tile / group / split options with the autotuner "
                          "(tuning time is reported as part of plan setup)")
-    ap.add_argument("--plan-opt", action="append", default=[], metavar="KEY=INT",
+    ap.add_argument("--plan-opt", action="append", default=[], metavar="KEY=VALUE",
                     help="plan tuning option passed to the operator (spread_group, "
-                         "interp_group, max_subproblem, spread_max)")
+                         "interp_group, max_subproblem, spread_max, spread_method as "
+                         "integers; tile as AxBxC)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); this build measures "
@@ -248,7 +249,9 @@ def main():
     smaps_np = gaussian_smaps(C, shape, seed=1)
     dev = torch.device("cuda", 0)
     t = time.perf_counter()
-    plan_opts = {k: int(v) for k, v in (o.split("=", 1) for o in args.plan_opt)}
+    plan_opts = {}
+    for k, v in (o.split("=", 1) for o in args.plan_opt):
+        plan_opts[k] = tuple(int(x) for x in v.split("x")) if k == "tile" else int(v)
     if args.autotune:
         plan_opts["autotune"] = True
     op = hipnufft.get_operator("hipnufft")(traj, shape, n_coils=C, smaps=smaps_np, **plan_opts)

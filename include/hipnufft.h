@@ -59,16 +59,21 @@ typedef struct hipnufft_opts {
   int spread_method;    /* HIPNUFFT_SPREAD_*: which spreading kernel         */
 } hipnufft_opts;
 
-/* Spreading kernels (all sum each cell in the same list order, so they
- * give the same bits).  AUTO: COLUMNS for 3D 8 x 8 x 16 tiles when
- * spread_group >= 16 and a set has >= 8 transforms, else BLOCKS for 3D
- * tiles with edges of 4, 8 or 16 cells (4, 8 or 16 blocks of 4 x 4 x 4
- * cells), else CELLS. */
+/* Spreading kernels.  CELLS, BLOCKS and COLUMNS sum each cell in the same
+ * list order with the same products, so they give the same bits; MATRIX
+ * sums the same entries in a fixed order on the matrix cores (the same
+ * result on every run, equal to the others to FP32 rounding).  AUTO:
+ * COLUMNS for 3D 8 x 8 x 16 tiles when spread_group >= 16 and a set has
+ * >= 8 transforms, else MATRIX for 3D tiles with edges of 4, 8 or 16 cells
+ * (4, 8 or 16 blocks of 4 x 4 x 4 cells), else CELLS. */
 enum {
   HIPNUFFT_SPREAD_AUTO = 0,
   HIPNUFFT_SPREAD_CELLS = 1,   /* thread per (column, 4-cell segment)   */
   HIPNUFFT_SPREAD_BLOCKS = 2,  /* wave per 4^3 blocks, untouched skipped */
-  HIPNUFFT_SPREAD_COLUMNS = 3  /* lane per 16-cell column, 16 transforms */
+  HIPNUFFT_SPREAD_COLUMNS = 3, /* lane per 16-cell column, 16 transforms */
+  HIPNUFFT_SPREAD_MATRIX = 4   /* 4^3 blocks on the matrix cores (FP32
+                                  MFMA); same order every run, equal to the
+                                  others to rounding, not bit for bit */
 };
 
 /* Fill *opts with defaults.  Replaces the cufinufft Plan **kwargs

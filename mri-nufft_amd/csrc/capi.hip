@@ -508,7 +508,7 @@ int hipnufft_plan_create(int type, int ndim, const int64_t* nmodes, int iflag, i
   p->interp_group = o.interp_group > 0 ? o.interp_group : (ndim == 3 ? 1 : 4);
   p->max_sub = o.max_subproblem > 0 ? o.max_subproblem : 1024;
   p->spread_max = o.spread_max > 0 ? o.spread_max : 4096;
-  if (o.spread_method < HIPNUFFT_SPREAD_AUTO || o.spread_method > HIPNUFFT_SPREAD_COLUMNS) {
+  if (o.spread_method < HIPNUFFT_SPREAD_AUTO || o.spread_method > HIPNUFFT_SPREAD_MATRIX) {
     free_plan(p);
     return set_error(HIPNUFFT_ERR_ARG, "unknown spread_method");
   }

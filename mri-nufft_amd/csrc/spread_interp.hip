@@ -302,6 +302,61 @@ __host__ __device__ constexpr int blk_record_floats(int W, int G) {
   return (kBlkHead + 2 * G + 3 * (W + 2 * kBlkPad) + 3) & ~3;
 }
 
+// Stage list entries [eb, eb + n) of a tile for the block and matrix
+// kernels: thread q < n writes entry eb + q's record at smem + q * PW.
+template <int G>
+__device__ __forceinline__ void stage_blk_entries(const SpreadArgs& a, float* smem, int eb, int n,
+                                                  int PW, int WP, const int* org, const int* ext,
+                                                  const int* nf, const int* nb, int t0, int Gv,
+                                                  float inv_hw, float beta) {
+  const int W = a.g.w;
+  if ((int)threadIdx.x < n) {
+    float* sp = smem + threadIdx.x * PW;
+    const int p = a.list_pts[eb + threadIdx.x];
+    uint32_t hit[3] = {0u, 0u, 0u};  // per axis: blocks along it the footprint meets
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const float off = a.g.off[d][p];
+      const int st = a.g.cell[d][p] - org[d];  // raw start, mod nf
+      float* wr = sp + kBlkHead + 2 * G + d * WP;
+      for (int j = 0; j < kBlkPad; ++j) wr[j] = 0.0f;
+      for (int j = 0; j < W; ++j) wr[kBlkPad + j] = es_eval(off - (float)j, inv_hw, beta);
+      for (int j = 0; j < kBlkPad; ++j) wr[kBlkPad + W + j] = 0.0f;
+      // offset of the block's first cell from the footprint start, in
+      // [-kBlkPad, W) when the block meets the footprint (nf >= W +
+      // kBlkPad; a block inside the grid has org + its origin < nf, so
+      // one wrap each way suffices); past the grid end: not met
+      uint32_t packed = 0u;
+      for (int bd = 0; bd < nb[d]; ++bd) {
+        const int bo = bd * kBlkE;
+        int x = bo - st;
+        if (x < 0) x += nf[d];
+        if (x >= nf[d]) x -= nf[d];
+        if (x >= nf[d] - kBlkPad) x -= nf[d];
+        if (bo < ext[d] && x < W) hit[d] |= 1u << bd;
+        const int o = kBlkHead + 2 * G + d * WP + kBlkPad + min(x, W - 1);  // < 256
+        packed |= (uint32_t)o << (8 * bd);
+      }
+      reinterpret_cast<uint32_t*>(sp)[1 + d] = packed;
+    }
+    uint32_t mask = 0u;
+    for (int b0 = 0; b0 < nb[0]; ++b0)
+      for (int b1 = 0; b1 < nb[1]; ++b1)
+        for (int b2 = 0; b2 < nb[2]; ++b2)
+          if ((hit[0] >> b0) & (hit[1] >> b1) & (hit[2] >> b2) & 1u)
+            mask |= 1u << ((b0 * nb[1] + b1) * nb[2] + b2);
+    reinterpret_cast<uint32_t*>(sp)[0] = mask;
+    float* sv = sp + kBlkHead;
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      float2 v = make_float2(0.f, 0.f);
+      if (g < Gv) v = a.in[(int64_t)(t0 + g) * a.in_stride + p];
+      sv[2 * g] = v.x;
+      sv[2 * g + 1] = v.y;
+    }
+  }
+}
+
 template <int G>
 __global__ __launch_bounds__(kThreads) void k_spread_blocks(SpreadArgs a) {
   extern __shared__ float smem[];
@@ -362,51 +417,7 @@ __global__ __launch_bounds__(kThreads) void k_spread_blocks(SpreadArgs a) {
   const int e1 = min(e0 + a.smax, a.list_start[key + 1]);
   for (int eb = e0; eb < e1; eb += kBlkChunk) {
     const int n = min(kBlkChunk, e1 - eb);
-    if ((int)threadIdx.x < n) {
-      float* sp = smem + threadIdx.x * PW;
-      const int p = a.list_pts[eb + threadIdx.x];
-      uint32_t hit[3] = {0u, 0u, 0u};  // per axis: blocks along it the footprint meets
-#pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        const float off = a.g.off[d][p];
-        const int st = a.g.cell[d][p] - org[d];  // raw start, mod nf
-        float* wr = sp + kBlkHead + 2 * G + d * WP;
-        for (int j = 0; j < kBlkPad; ++j) wr[j] = 0.0f;
-        for (int j = 0; j < W; ++j) wr[kBlkPad + j] = es_eval(off - (float)j, inv_hw, beta);
-        for (int j = 0; j < kBlkPad; ++j) wr[kBlkPad + W + j] = 0.0f;
-        // offset of the block's first cell from the footprint start, in
-        // [-kBlkPad, W) when the block meets the footprint (nf >= W +
-        // kBlkPad; a block inside the grid has org + its origin < nf, so
-        // one wrap each way suffices); past the grid end: not met
-        uint32_t packed = 0u;
-        for (int bd = 0; bd < nb[d]; ++bd) {
-          const int bo = bd * kBlkE;
-          int x = bo - st;
-          if (x < 0) x += nf[d];
-          if (x >= nf[d]) x -= nf[d];
-          if (x >= nf[d] - kBlkPad) x -= nf[d];
-          if (bo < ext[d] && x < W) hit[d] |= 1u << bd;
-          const int o = kBlkHead + 2 * G + d * WP + kBlkPad + min(x, W - 1);  // < 256
-          packed |= (uint32_t)o << (8 * bd);
-        }
-        reinterpret_cast<uint32_t*>(sp)[1 + d] = packed;
-      }
-      uint32_t mask = 0u;
-      for (int b0 = 0; b0 < nb[0]; ++b0)
-        for (int b1 = 0; b1 < nb[1]; ++b1)
-          for (int b2 = 0; b2 < nb[2]; ++b2)
-            if ((hit[0] >> b0) & (hit[1] >> b1) & (hit[2] >> b2) & 1u)
-              mask |= 1u << ((b0 * nb[1] + b1) * nb[2] + b2);
-      reinterpret_cast<uint32_t*>(sp)[0] = mask;
-      float* sv = sp + kBlkHead;
-#pragma unroll
-      for (int g = 0; g < G; ++g) {
-        float2 v = make_float2(0.f, 0.f);
-        if (g < Gv) v = a.in[(int64_t)(t0 + g) * a.in_stride + p];
-        sv[2 * g] = v.x;
-        sv[2 * g + 1] = v.y;
-      }
-    }
+    stage_blk_entries<G>(a, smem, eb, n, PW, WP, org, ext, nf, nb, t0, Gv, inv_hw, beta);
     __syncthreads();
     // per 64 staged entries: lane i reads the header of entry base + i, a
     // ballot marks the entries that touch one of the wave's blocks, and the
@@ -469,6 +480,152 @@ __global__ __launch_bounds__(kThreads) void k_spread_blocks(SpreadArgs a) {
 #pragma unroll
     for (int g = 0; g < G; ++g)
       if (g < Gv) a.grid[(int64_t)(tb + t0 + g) * a.grid_stride + gi] = acc[bi][g];
+  }
+}
+
+// Matrix-core spreading for the tiles of the block kernel, 8 transforms per
+// workgroup.  For one 4 x 4 x 4 block and four entries touching it, the sums
+// of the block's cells are a 64 x 4 by 4 x 16 product: cells by entries
+// (the weight products w0 w1 w2) times entries by values (the strengths, 8
+// transforms x re / im).  Each slice of 16 cells (fixed first offset) is one
+// v_mfma_f32_16x16x4f32: lane l supplies A[l % 16][l / 16] (cell l % 16 of
+// the slice, entry l / 16), B[l / 16][l % 16] (entry l / 16, value l % 16)
+// and holds D[4 (l / 16) + r][l % 16] in register r.  A wave takes the
+// entries touching each of its blocks four at a time in list order (a
+// ballot per 64 staged entries), so every cell's sum has the same order on
+// every run: deterministic, equal to the other kernels to rounding.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(kThreads, 4) void k_spread_mfma(SpreadArgs a) {
+  constexpr int G = 8;
+  extern __shared__ float smem[];
+  const int W = a.g.w;
+  const int WP = W + 2 * kBlkPad;
+  const int PW = blk_record_floats(W, G);
+  const int item = a.item0 + blockIdx.x;
+  const int key = a.it_tile[item];
+  const int set = key / a.g.nbins;
+  const int tile = key - set * a.g.nbins;
+  const int slot = a.it_slot[item];
+  const int t0 = blockIdx.y * G;
+  const int tb = (set - a.k0) * a.ts;
+  const int Gv = min(G, a.ts - t0);
+  const float inv_hw = 2.0f / W, beta = a.g.beta;
+
+  int tc[3], org[3], ext[3], td[3], nf[3], nb[3];
+  tile_coords<3>(tile, a.g.nt, tc);
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    td[d] = a.g.tile[d];
+    nf[d] = a.g.nf[d];
+    org[d] = tc[d] * td[d];
+    ext[d] = min(td[d], nf[d] - org[d]);
+    nb[d] = td[d] / kBlkE;
+  }
+  const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x / 64);
+  const int lane = (int)threadIdx.x & 63;
+  const int kq = lane >> 4;               // entry of the group (A column, B row)
+  const int cy = (lane >> 2) & 3, cz = lane & 3;  // A row: cell (slice, cy, cz)
+  const int nbw = nb[0] * nb[1] * nb[2] / 4;
+  int bidx[kBlkPerWave][3];
+  uint32_t bbit[kBlkPerWave];
+#pragma unroll
+  for (int bi = 0; bi < kBlkPerWave; ++bi) {
+    const int b = wave + 4 * bi;
+    bidx[bi][2] = b % nb[2];
+    bidx[bi][1] = (b / nb[2]) % nb[1];
+    bidx[bi][0] = b / (nb[2] * nb[1]);
+    bool live = bi < nbw;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) live = live && bidx[bi][d] * kBlkE < ext[d];
+    bbit[bi] = live ? 1u << b : 0u;
+  }
+
+  f32x4 acc[kBlkPerWave][kBlkE];
+#pragma unroll
+  for (int bi = 0; bi < kBlkPerWave; ++bi)
+#pragma unroll
+    for (int s = 0; s < kBlkE; ++s) acc[bi][s] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+  const int e0 = a.it_begin[item];
+  const int e1 = min(e0 + a.smax, a.list_start[key + 1]);
+  for (int eb = e0; eb < e1; eb += kBlkChunk) {
+    const int n = min(kBlkChunk, e1 - eb);
+    stage_blk_entries<G>(a, smem, eb, n, PW, WP, org, ext, nf, nb, t0, Gv, inv_hw, beta);
+    __syncthreads();
+    for (int base = 0; base < n; base += 64) {
+      const int qi = base + lane;
+      const uint32_t hm = qi < n ? reinterpret_cast<const uint32_t*>(smem + qi * PW)[0] : 0u;
+#pragma unroll
+      for (int bi = 0; bi < kBlkPerWave; ++bi) {
+        if (!bbit[bi]) continue;  // uniform
+        uint64_t todo = __ballot((hm & bbit[bi]) != 0u);
+        while (todo) {  // uniform: four entries per step, list order
+          int ek[4];
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            ek[k] = todo ? __builtin_ctzll(todo) : -1;
+            todo &= todo - 1;
+          }
+          const int e = kq == 0 ? ek[0] : kq == 1 ? ek[1] : kq == 2 ? ek[2] : ek[3];
+          float av[kBlkE] = {0.f, 0.f, 0.f, 0.f};
+          float bv = 0.f;
+          if (e >= 0) {
+            const float* sp = smem + (base + e) * PW;
+            const uint32_t* hd = reinterpret_cast<const uint32_t*>(sp);
+            const int ox = (hd[1] >> (8 * bidx[bi][0])) & 255;
+            const int oy = (hd[2] >> (8 * bidx[bi][1])) & 255;
+            const int oz = (hd[3] >> (8 * bidx[bi][2])) & 255;
+            const float wyz = sp[oy + cy] * sp[oz + cz];
+#pragma unroll
+            for (int sx = 0; sx < kBlkE; ++sx) av[sx] = sp[ox + sx] * wyz;
+            bv = sp[kBlkHead + (lane & 15)];
+          }
+#pragma unroll
+          for (int sx = 0; sx < kBlkE; ++sx)
+            acc[bi][sx] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[sx], bv, acc[bi][sx], 0, 0, 0);
+        }
+      }
+    }
+    __syncthreads();
+  }
+
+  // register r of lane l holds value l % 16 (transform (l % 16) / 2, re / im
+  // by parity) of cell (slice, l / 16, r); lanes 2g and 2g + 1 swap halves
+  // so that each writes two whole complex cells
+  const int TC = td[0] * td[1] * td[2];
+  const int g = (lane & 15) >> 1;
+  const bool odd = lane & 1;
+  const int ry = lane >> 4;
+#pragma unroll
+  for (int bi = 0; bi < kBlkPerWave; ++bi) {
+    if (!bbit[bi]) continue;
+#pragma unroll
+    for (int sx = 0; sx < kBlkE; ++sx) {
+      float2 out[2];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        // even lane keeps cells r = 0, 1, odd lane cells r = 2, 3
+        const float mine = odd ? acc[bi][sx][2 + h] : acc[bi][sx][h];
+        const float give = odd ? acc[bi][sx][h] : acc[bi][sx][2 + h];
+        const float other = __shfl_xor(give, 1);
+        out[h] = odd ? make_float2(other, mine) : make_float2(mine, other);
+      }
+      if (g >= Gv) continue;
+      const int c0 = bidx[bi][0] * kBlkE + sx, c1 = bidx[bi][1] * kBlkE + ry;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int c2 = bidx[bi][2] * kBlkE + (odd ? 2 : 0) + h;
+        if (c0 >= ext[0] || c1 >= ext[1] || c2 >= ext[2]) continue;
+        if (slot >= 0) {
+          const int r = (c0 * td[1] + c1) * td[2] + c2;
+          a.part[((int64_t)slot * a.ntrans + tb + t0 + g) * TC + r] = out[h];
+        } else {
+          const int64_t gi = ((int64_t)(org[0] + c0) * nf[1] + org[1] + c1) * nf[2] + org[2] + c2;
+          a.grid[(int64_t)(tb + t0 + g) * a.grid_stride + gi] = out[h];
+        }
+      }
+    }
   }
 }
 
@@ -1122,9 +1279,9 @@ int launch_spread_sorted(const hipnufft_plan_s* p, const float2* in, SetRange r,
       return set_error(HIPNUFFT_ERR_ARG, "spread tile has more than 256 column segments of 4 cells");
   }
   // kernel choice: column owners for 3D 8 x 8 x 16 tiles when asked for
-  // (spread_group >= 16 and at least 8 transforms per set), else block
-  // owners for 3D tiles of 4^3-cell blocks (edges 4, 8, 16), else the
-  // cell-owner kernel
+  // (spread_group >= 16 and at least 8 transforms per set), else the
+  // matrix-core kernel for 3D tiles of 4^3-cell blocks (edges 4, 8, 16),
+  // else the cell-owner kernel
   const bool cols_ok = p->ndim == 3 && p->tile[0] == 8 && p->tile[1] == 8 &&
                        p->tile[2] == kColsZ && r.ts >= 8;
   bool blocks_ok = p->ndim == 3;
@@ -1140,15 +1297,31 @@ int launch_spread_sorted(const hipnufft_plan_s* p, const float2* in, SetRange r,
   int method = p->spread_method;
   if (method == HIPNUFFT_SPREAD_AUTO)
     method = (cols_ok && p->spread_group >= 16) ? HIPNUFFT_SPREAD_COLUMNS
-             : blocks_ok                        ? HIPNUFFT_SPREAD_BLOCKS
+             : blocks_ok                        ? HIPNUFFT_SPREAD_MATRIX
                                                 : HIPNUFFT_SPREAD_CELLS;
   if (method == HIPNUFFT_SPREAD_COLUMNS && !cols_ok)
     return set_error(HIPNUFFT_ERR_ARG,
                      "column spreading needs 3D 8 x 8 x 16 tiles and >= 8 transforms per set");
-  if (method == HIPNUFFT_SPREAD_BLOCKS && !blocks_ok)
+  if ((method == HIPNUFFT_SPREAD_BLOCKS || method == HIPNUFFT_SPREAD_MATRIX) && !blocks_ok)
     return set_error(HIPNUFFT_ERR_ARG,
                      "block spreading needs 3D tiles of 4, 8 or 16 blocks of 4^3 cells, "
                      "at most 4 along an axis");
+  if (method == HIPNUFFT_SPREAD_MATRIX) {
+    a.G = 8;
+    const size_t lds = sizeof(float) * (size_t)kBlkChunk * blk_record_floats(p->w, 8);
+    if (nitems > 0) {
+      const dim3 grd((unsigned)nitems, (unsigned)((r.ts + 7) / 8));
+      StageScope timer(p, kStageSpread);
+      hipLaunchKernelGGL(k_spread_mfma, grd, dim3(kThreads), lds, p->stream, a);
+      HN_LAUNCH_CHECK();
+    }
+    if (ndense > 0) {
+      const dim3 rg((unsigned)ndense, (unsigned)r.ts);
+      hipLaunchKernelGGL(k_spread_reduce<3>, rg, dim3(kThreads), 0, p->stream, a);
+      HN_LAUNCH_CHECK();
+    }
+    return 0;
+  }
   if (method == HIPNUFFT_SPREAD_BLOCKS) {
     int G = p->spread_group > 0 ? p->spread_group : 8;
     G = G >= 8 ? 8 : G >= 4 ? 4 : G >= 2 ? 2 : 1;

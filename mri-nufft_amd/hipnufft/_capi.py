@@ -138,7 +138,7 @@ def current_stream() -> int:
     return torch.cuda.current_stream().cuda_stream
 
 
-SPREAD_METHODS = {"auto": 0, "cells": 1, "blocks": 2, "columns": 3}
+SPREAD_METHODS = {"auto": 0, "cells": 1, "blocks": 2, "columns": 3, "matrix": 4}
 
 
 def spread_method_id(m):

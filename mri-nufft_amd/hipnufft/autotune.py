@@ -6,7 +6,8 @@ plan options and keeps the fastest, by coordinate search:
 
 1. tile shape (which sets both the bin sort and the spread / interp work
    granularity);
-2. in 3D, the spreading kernel (``spread_method``: cell or block owners);
+2. in 3D, the spreading kernel (``spread_method``: cell owners, block
+   owners or the matrix-core kernel);
 3. transforms accumulated per spreading workgroup (``spread_group``);
 4. transforms gathered per interpolation workgroup (``interp_group``);
 5. dense-bin splitting (``max_subproblem``, ``spread_max``).
@@ -142,7 +143,7 @@ def autotune(coords, shape, n_trans=1, eps=1e-6, upsampfac=2.0, reps=3, budget_s
     best_ms = timed(best)
     stages = [
         ("tile", tile_candidates(ndim, w)),
-        ("spread_method", [1, 2] if ndim == 3 else []),
+        ("spread_method", [1, 2, 4] if ndim == 3 else []),
         ("spread_group", [4, 8, 16]),
         ("interp_group", [1, 2, 4] if ndim < 3 else [1]),
         ("max_subproblem", [512, 1024, 2048]),

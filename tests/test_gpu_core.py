@@ -161,8 +161,9 @@ def test_dot_fp64():
 def test_spread_kernels_bit_identical(T):
     """3D 8x8x16 tiles: the cell-owner, block-owner and (>= 8 transforms)
     column-owner spreading kernels sum the same list entries in the same
-    order with the same weights: bit-identical grids.  Includes split dense
-    tiles (spread_max) and a partially filled last transform group."""
+    order with the same weights: bit-identical grids; the matrix-core kernel
+    agrees to FP32 rounding.  Includes split dense tiles (spread_max) and a
+    partially filled last transform group."""
     from hipnufft import _capi
 
     traj = np.concatenate([rand_traj(3000, 3, 41), rand_traj(2500, 3, 42) * 0.06])
@@ -171,7 +172,7 @@ def test_spread_kernels_bit_identical(T):
     y = torch.from_numpy(rand_cplx((T, traj.shape[0]), seed=43)).to(DEV)
     methods = ["cells", "blocks"] + (["columns"] if T >= 8 else [])
     outs = {}
-    for m in methods:
+    for m in methods + ["matrix"]:
         p = _capi.RawPlan(shape, n_trans=T, spread_group=16 if m == "columns" else 8,
                           spread_max=300, spread_method=m)
         p.setpts(om)
@@ -182,6 +183,8 @@ def test_spread_kernels_bit_identical(T):
     torch.cuda.synchronize()
     for m in methods[1:]:
         assert torch.equal(outs["cells"], outs[m]), m
+    # matrix cores: same entries, FP32 sums in another order
+    assert rel_l2(outs["matrix"].cpu().numpy(), outs["cells"].cpu().numpy()) < 1e-6
     ref = nufft.Plan(om.cpu().numpy(), shape, eps=1e-6)
     for t in (0, T - 1):
         exp = ref.execute_adjoint(y[t].cpu().numpy().astype(np.complex128))
@@ -205,7 +208,7 @@ def test_spread_blocks_other_tiles(tile, shape, eps):
     T = 5
     y = torch.from_numpy(rand_cplx((T, traj.shape[0]), seed=46)).to(DEV)
     outs = {}
-    for m in ("cells", "blocks"):
+    for m in ("cells", "blocks", "matrix"):
         p = _capi.RawPlan(shape, n_trans=T, eps=eps, tile=tile, spread_max=200, spread_method=m)
         p.setpts(om)
         assert p.info()["tile"] == tile
@@ -214,6 +217,7 @@ def test_spread_blocks_other_tiles(tile, shape, eps):
         outs[m] = x
     torch.cuda.synchronize()
     assert torch.equal(outs["cells"], outs["blocks"])
+    assert rel_l2(outs["matrix"].cpu().numpy(), outs["cells"].cpu().numpy()) < 1e-6
     ref = nufft.Plan(om.cpu().numpy(), shape, eps=eps)
     exp = ref.execute_adjoint(y[T - 1].cpu().numpy().astype(np.complex128))
     assert rel_l2(outs["blocks"][T - 1].cpu().numpy(), exp) < 10 * eps
