@@ -57,6 +57,8 @@ typedef struct hipnufft_opts {
   int spread_max;       /* max list entries per spread work item (dense
                            tiles are split and their partials summed)        */
   int spread_method;    /* HIPNUFFT_SPREAD_*: which spreading kernel         */
+  int interp_span;      /* bins along the last axis staged together by one
+                           interpolation workgroup (0: 2 in 3D, else 1)      */
 } hipnufft_opts;
 
 /* Spreading kernels.  CELLS, BLOCKS and COLUMNS sum each cell in the same

@@ -438,6 +438,7 @@ int hipnufft_default_opts(hipnufft_opts* o) {
   o->deterministic = 0;
   o->spread_max = 0;
   o->spread_method = HIPNUFFT_SPREAD_AUTO;
+  o->interp_span = 0;
   return 0;
 }
 
@@ -513,6 +514,8 @@ int hipnufft_plan_create(int type, int ndim, const int64_t* nmodes, int iflag, i
     return set_error(HIPNUFFT_ERR_ARG, "unknown spread_method");
   }
   p->spread_method = o.spread_method;
+  p->interp_span = o.interp_span > 0 ? o.interp_span : (ndim == 3 ? 2 : 1);
+  p->interp_span = (int)std::min<int64_t>(p->interp_span, p->nt[2]);
 
   int rc = 0;
   // deconvolution tables (padded axes hold a single 1)

@@ -115,6 +115,7 @@ struct hipnufft_plan_s {
   int64_t nbins = 1;           // tiles per point set
   int spread_group = 1, interp_group = 1, max_sub = 1024;
   int spread_method = 0;  // HIPNUFFT_SPREAD_*
+  int interp_span = 1;    // bins along the last axis per interpolation work item
   // point sets (trajectories): set_off[k] (host and device, nsets + 1) and the
   // first interp item / spread item / dense tile of each set (host, nsets + 1)
   int nsets = 1;

@@ -255,20 +255,34 @@ __global__ __launch_bounds__(kThreads) void k_permute(PermArgs a) {
   }
 }
 
-__global__ __launch_bounds__(kThreads) void k_sub_count(const int32_t* bs, int64_t nbins, int S,
+// Interpolation work items: a lead bin (last-axis tile index a multiple of
+// span) owns the points of its span consecutive bins along the last axis
+// (adjacent keys, so one contiguous range), split into items of at most S
+// points; the other bins own none.
+__device__ __forceinline__ int span_points(const int32_t* bs, int64_t b, uint32_t nbins, int nt2,
+                                           int span) {
+  const int t2 = (int)((uint64_t)b % nbins) % nt2;
+  if (t2 % span) return 0;
+  const int m = min(span, nt2 - t2);
+  return bs[b + m] - bs[b];
+}
+
+__global__ __launch_bounds__(kThreads) void k_sub_count(const int32_t* bs, int64_t nkeys,
+                                                        uint32_t nbins, int nt2, int span, int S,
                                                         int32_t* cnt) {
   const int64_t b = (int64_t)blockIdx.x * kThreads + threadIdx.x;
-  if (b >= nbins) return;
-  const int n = bs[b + 1] - bs[b];
+  if (b >= nkeys) return;
+  const int n = span_points(bs, b, nbins, nt2, span);
   cnt[b] = (n + S - 1) / S;
 }
 
 __global__ __launch_bounds__(kThreads) void k_sub_emit(const int32_t* bs, const int32_t* soff,
-                                                       int64_t nbins, int S, int32_t* st,
+                                                       int64_t nkeys, uint32_t nbins, int nt2,
+                                                       int span, int S, int32_t* st,
                                                        int32_t* sb) {
   const int64_t b = (int64_t)blockIdx.x * kThreads + threadIdx.x;
-  if (b >= nbins) return;
-  const int n = bs[b + 1] - bs[b];
+  if (b >= nkeys) return;
+  const int n = span_points(bs, b, nbins, nt2, span);
   const int k = (n + S - 1) / S;
   for (int s = 0; s < k; ++s) {
     st[soff[b] + s] = (int32_t)b;
@@ -656,13 +670,14 @@ int build_points(hipnufft_plan_s* p, int nsets, const int64_t* Ms, const float* 
     HN_LAUNCH_CHECK();
   }
 
-  // interp work items: split dense bins into chunks of max_sub points
+  // interp work items: the points of interp_span bins along the last axis,
+  // split into chunks of max_sub points
   {
     int32_t* cnt = nullptr;
     HN_HIP(hipMallocAsync((void**)&cnt, sizeof(int32_t) * nkeys, s));
     const unsigned g = (unsigned)((nkeys + kThreads - 1) / kThreads);
-    hipLaunchKernelGGL(k_sub_count, dim3(g), dim3(kThreads), 0, s, p->bin_start, nkeys, p->max_sub,
-                       cnt);
+    hipLaunchKernelGGL(k_sub_count, dim3(g), dim3(kThreads), 0, s, p->bin_start, nkeys,
+                       (uint32_t)p->nbins, p->nt[2], p->interp_span, p->max_sub, cnt);
     HN_LAUNCH_CHECK();
     int64_t nsub = 0;
     if ((rc = scan_exclusive(cnt, cnt, nkeys, s, &nsub))) return rc;
@@ -675,7 +690,8 @@ int build_points(hipnufft_plan_s* p, int nsets, const int64_t* Ms, const float* 
     }
     p->nsub = nsub;
     hipLaunchKernelGGL(k_sub_emit, dim3(g), dim3(kThreads), 0, s, p->bin_start, cnt, nkeys,
-                       p->max_sub, p->sub_tile, p->sub_begin);
+                       (uint32_t)p->nbins, p->nt[2], p->interp_span, p->max_sub, p->sub_tile,
+                       p->sub_begin);
     HN_LAUNCH_CHECK();
     HN_HIP(hipFreeAsync(cnt, s));
   }

@@ -88,6 +88,7 @@ struct InterpArgs {
   int wrap1;
   int stage_lanes, stage_rows;
   uint32_t lane_mag;
+  int span;  // bins along the last axis per work item (region: span tiles + halo)
 };
 
 __device__ __forceinline__ int mdiv(int c, uint32_t magic) {
@@ -820,7 +821,7 @@ __device__ __forceinline__ int wrap_coord(int x, int nfd, int wrap1) {
 // the rows, kStageBatch loads in flight before their LDS stores.  The table
 // is built here and synchronised before the loads; the caller synchronises
 // after.
-template <int D, int W>
+template <int D, int W, int NT = kThreads>
 __device__ __forceinline__ void stage_region(const InterpArgs& a, float2* lds, int tile, int t0,
                                              int G, int* rorg, int* R, int& RC) {
   constexpr int kStageBatch = 24;
@@ -829,7 +830,7 @@ __device__ __forceinline__ void stage_region(const InterpArgs& a, float2* lds, i
   RC = 1;
 #pragma unroll
   for (int d = 0; d < D; ++d) {
-    R[d] = a.g.tile[d] + W;
+    R[d] = (d == D - 1 ? a.span : 1) * a.g.tile[d] + W;
     rorg[d] = tc[d] * a.g.tile[d] - W / 2;
     RC *= R[d];
   }
@@ -838,16 +839,16 @@ __device__ __forceinline__ void stage_region(const InterpArgs& a, float2* lds, i
     for (int gg = 0; gg < G; ++gg) {
       const float2* src = a.grid + (int64_t)(t0 + gg) * a.grid_stride;
       float2* dst = lds + gg * RC;
-      for (int c0 = threadIdx.x; c0 < RC; c0 += kStageBatch * kThreads) {
+      for (int c0 = threadIdx.x; c0 < RC; c0 += kStageBatch * NT) {
         float2 v[kStageBatch];
 #pragma unroll
         for (int u = 0; u < kStageBatch; ++u) {
-          const int c = c0 + u * kThreads;
+          const int c = c0 + u * NT;
           if (c < RC) v[u] = src[wrap_coord(rorg[0] + c, nfd, a.wrap1)];
         }
 #pragma unroll
         for (int u = 0; u < kStageBatch; ++u) {
-          const int c = c0 + u * kThreads;
+          const int c = c0 + u * NT;
           if (c < RC) dst[c] = v[u];
         }
       }
@@ -857,7 +858,7 @@ __device__ __forceinline__ void stage_region(const InterpArgs& a, float2* lds, i
     const int RL = R[DL];
     const int NR = D == 2 ? R[0] : R[0] * R[1];
     int* rowoff = reinterpret_cast<int*>(lds + G * RC);
-    for (int r = threadIdx.x; r < NR; r += kThreads) {
+    for (int r = threadIdx.x; r < NR; r += NT) {
       int off;
       if constexpr (D == 2) {
         off = wrap_coord(rorg[0] + r, a.g.nf[0], a.wrap1);
@@ -870,7 +871,7 @@ __device__ __forceinline__ void stage_region(const InterpArgs& a, float2* lds, i
       rowoff[r] = off * a.g.nf[DL];
     }
     __syncthreads();
-    // L = min(RL, kThreads) lanes per row, P = kThreads / L rows per pass
+    // L = min(RL, NT) lanes per row, P = NT / L rows per pass (set by the host)
     const int L = a.stage_lanes, P = a.stage_rows;
     const int rs = mdiv((int)threadIdx.x, a.lane_mag);
     const int lz = (int)threadIdx.x - rs * L;
@@ -900,6 +901,14 @@ __device__ __forceinline__ void stage_region(const InterpArgs& a, float2* lds, i
   }
 }
 
+// bins whose points a work item led by tile covers: span along the last
+// axis, fewer at the grid end
+template <int D>
+__device__ __forceinline__ int span_bins(const InterpArgs& a, int tile) {
+  const int nl = a.g.nt[D - 1];
+  return min(a.span, nl - tile % nl);
+}
+
 template <int D, int W>
 __device__ __forceinline__ void point_setup(const InterpArgs& a, int p, const int* rorg,
                                             int* ls, float (*wt)[W]) {
@@ -909,7 +918,8 @@ __device__ __forceinline__ void point_setup(const InterpArgs& a, int p, const in
     int l = a.g.cell[d][p] - rorg[d];
     if (l < 0) l += a.g.nf[d];
     if (l >= a.g.nf[d]) l -= a.g.nf[d];
-    l = min(max(l, 0), a.g.tile[d]);  // memory safety; a sorted point always satisfies it
+    // memory safety; a sorted point always satisfies it
+    l = min(max(l, 0), (d == D - 1 ? a.span : 1) * a.g.tile[d]);
     ls[d] = l;
     const float off = a.g.off[d][p];
 #pragma unroll
@@ -917,23 +927,23 @@ __device__ __forceinline__ void point_setup(const InterpArgs& a, int p, const in
   }
 }
 
-template <int D, int W>
-__global__ __launch_bounds__(kThreads) void k_interp(InterpArgs a) {
+template <int D, int W, int NT>
+__global__ __launch_bounds__(NT) void k_interp(InterpArgs a) {
   extern __shared__ float2 lds[];
   const int sub = a.sub0 + blockIdx.x;
   const int key = a.sub_tile[sub];
   const int set = key / a.g.nbins;
   const int tile = key - set * a.g.nbins;
   const int pb = a.sub_begin[sub];
-  const int pe = min(pb + a.max_sub, a.bin_start[key + 1]);
+  const int pe = min(pb + a.max_sub, a.bin_start[key + span_bins<D>(a, tile)]);
   const int t0 = blockIdx.y * a.G;       // first transform of the set handled here
   const int G = min(a.G, a.ts - t0);
   const int tb = (set - a.k0) * a.ts;    // chunk transform of the set's first
   int rorg[3], R[3], RC;
-  stage_region<D, W>(a, lds, tile, tb + t0, G, rorg, R, RC);
+  stage_region<D, W, NT>(a, lds, tile, tb + t0, G, rorg, R, RC);
   __syncthreads();
 
-  for (int p = pb + threadIdx.x; p < pe; p += kThreads) {
+  for (int p = pb + threadIdx.x; p < pe; p += NT) {
     int ls[3];
     float wt[D][W];
     point_setup<D, W>(a, p, rorg, ls, wt);
@@ -1011,7 +1021,7 @@ __global__ __launch_bounds__(kThreads) void k_grad_traj(InterpArgs a) {
   const int set = key / a.g.nbins;
   const int tile = key - set * a.g.nbins;
   const int pb = a.sub_begin[sub];
-  const int pe = min(pb + a.max_sub, a.bin_start[key + 1]);
+  const int pe = min(pb + a.max_sub, a.bin_start[key + span_bins<D>(a, tile)]);
   const int G = a.G;  // transforms [tg0, tg0 + G) of the set in one workgroup
   const int tb = (set - a.k0) * a.ts + a.tg0;
   int rorg[3], R[3], RC;
@@ -1174,26 +1184,47 @@ static uint32_t magic_of(int64_t d) {  // ceil(2^32 / d), d >= 2
   return (uint32_t)((((uint64_t)1 << 32) + (uint64_t)d - 1) / (uint64_t)d);
 }
 
-static void set_region_magic(const hipnufft_plan_s* p, InterpArgs* a) {
+// extent of an interpolation region along axis d (3-padded index): the
+// work item's tiles (interp_span of them along the last axis) plus the
+// kernel width
+static int region_extent(const hipnufft_plan_s* p, int d) {
+  return (d == 2 ? p->interp_span : 1) * p->tile[d] + p->w;
+}
+
+static void set_region_magic(const hipnufft_plan_s* p, InterpArgs* a, int nt = kThreads) {
   const int o = 3 - p->ndim;
   a->wrap1 = 1;
   for (int d = 0; d < p->ndim; ++d) {
-    const int R = p->tile[o + d] + p->w;  // >= 3
+    const int R = region_extent(p, o + d);  // >= 3
     a->rmag[d] = magic_of(R);
     if (p->nf[o + d] < R) a->wrap1 = 0;
   }
-  const int RL = p->tile[2] + p->w;
-  a->stage_lanes = std::min(RL, kThreads);
-  a->stage_rows = kThreads / a->stage_lanes;
+  const int RL = region_extent(p, 2);
+  a->stage_lanes = std::min(RL, nt);
+  a->stage_rows = nt / a->stage_lanes;
   a->lane_mag = magic_of(a->stage_lanes);
+  a->span = p->interp_span;
 }
 
 // LDS of an interpolation workgroup staging G transforms: the regions and,
 // in 2D / 3D, the table of row offsets after them
 static size_t region_lds_bytes(const hipnufft_plan_s* p, int G) {
-  const size_t cells = (size_t)tile_cells(p, p->w);
-  const size_t rows = p->ndim == 1 ? 0 : cells / (size_t)(p->tile[2] + p->w);
+  size_t cells = 1;
+  for (int d = 3 - p->ndim; d < 3; ++d) cells *= (size_t)region_extent(p, d);
+  const size_t rows = p->ndim == 1 ? 0 : cells / (size_t)region_extent(p, 2);
   return sizeof(float2) * cells * (size_t)G + sizeof(int) * rows;
+}
+
+// one transform's region may use all of a CU's LDS; several transforms per
+// workgroup only while they fit the 64 KiB that keeps workgroups co-resident
+constexpr size_t kMaxLdsRegion = 160 * 1024;
+
+template <typename K>
+static int allow_lds(K kernel, size_t lds) {
+  if (lds <= kMaxLds) return 0;
+  HN_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  return 0;
 }
 
 static int check_range(const hipnufft_plan_s* p, SetRange r) {
@@ -1419,17 +1450,26 @@ int launch_interp(const hipnufft_plan_s* p, const float2* grid, int64_t grid_str
   a.ts = r.ts;
   a.scale = scale;
   a.mode = mode;
-  set_region_magic(p, &a);
-  if (region_lds_bytes(p, 1) > kMaxLds)
+  if (region_lds_bytes(p, 1) > kMaxLdsRegion)
     return set_error(HIPNUFFT_ERR_ARG, "interp tile too large for LDS");
   int G = std::min(p->interp_group, r.ts);
   while (G > 1 && region_lds_bytes(p, G) > kMaxLds) --G;
   a.G = G;
   const size_t lds = region_lds_bytes(p, G);
+  // regions past 64 KiB leave room for two workgroups per CU: 512 threads
+  // each keep as many waves resident as four 256-thread workgroups
+  const int nt = lds > kMaxLds ? 2 * kThreads : kThreads;
+  set_region_magic(p, &a, nt);
   const dim3 grd((unsigned)nsub, (unsigned)((r.ts + G - 1) / G));
   StageScope timer(p, kStageInterp);
   HN_DIM_SWITCH(p->ndim, HN_WIDTH_SWITCH(p->w, {
-    hipLaunchKernelGGL((k_interp<DD, WW>), grd, dim3(kThreads), lds, p->stream, a);
+    if (nt == kThreads) {
+      hipLaunchKernelGGL((k_interp<DD, WW, kThreads>), grd, dim3(kThreads), lds, p->stream, a);
+    } else {
+      if ((rc = allow_lds(k_interp<DD, WW, 2 * kThreads>, lds))) return rc;
+      hipLaunchKernelGGL((k_interp<DD, WW, 2 * kThreads>), grd, dim3(2 * kThreads), lds,
+                         p->stream, a);
+    }
   }));
   HN_LAUNCH_CHECK();
   return 0;
@@ -1460,7 +1500,7 @@ int launch_grad_traj(const hipnufft_plan_s* p, const float2* grid, int64_t grid_
   for (int d = 0; d < p->ndim; ++d) a.dscale[d] = (float)((double)p->nf[o + d] / (2.0 * M_PI));
   // each workgroup handles the transforms of its set that fit LDS; the
   // launches accumulate into gtraj in stream order
-  if (region_lds_bytes(p, 1) > kMaxLds)
+  if (region_lds_bytes(p, 1) > kMaxLdsRegion)
     return set_error(HIPNUFFT_ERR_ARG, "interp tile too large for LDS");
   int G = 1;
   while (region_lds_bytes(p, G + 1) <= kMaxLds) ++G;
@@ -1471,6 +1511,7 @@ int launch_grad_traj(const hipnufft_plan_s* p, const float2* grid, int64_t grid_
     const size_t lds = region_lds_bytes(p, Tg);
     StageScope timer(p, kStageGradTraj);
     HN_DIM_SWITCH(p->ndim, HN_WIDTH_SWITCH(p->w, {
+      if ((rc = allow_lds(k_grad_traj<DD, WW>, lds))) return rc;
       hipLaunchKernelGGL((k_grad_traj<DD, WW>), dim3((unsigned)nsub), dim3(kThreads), lds,
                          p->stream, a);
     }));

@@ -47,6 +47,7 @@ class Opts(ctypes.Structure):
         ("deterministic", ctypes.c_int),
         ("spread_max", ctypes.c_int),
         ("spread_method", ctypes.c_int),
+        ("interp_span", ctypes.c_int),
     ]
 
 
@@ -161,7 +162,7 @@ class RawPlan:
     def __init__(self, shape, n_trans=1, eps=1e-6, isign=-1, upsampfac=2.0,
                  spreadinterponly=False, tile=None, spread_group=0,
                  interp_group=0, max_subproblem=0, spread_max=0, spread_method=0,
-                 stream=None):
+                 interp_span=0, stream=None):
         L = lib()
         self.shape = tuple(int(s) for s in shape)
         self.ndim = len(self.shape)
@@ -180,6 +181,7 @@ class RawPlan:
         o.max_subproblem = int(max_subproblem)
         o.spread_max = int(spread_max)
         o.spread_method = spread_method_id(spread_method)
+        o.interp_span = int(interp_span)
         self.opts = o
         nm = (_I64 * self.ndim)(*self.shape)
         h = _P()

@@ -12,10 +12,7 @@ plan options and keeps the fastest, by coordinate search:
 4. transforms gathered per interpolation workgroup (``interp_group``);
 5. dense-bin splitting (``max_subproblem``, ``spread_max``).
 
-Only candidates the kernels accept are tried: a spreading tile needs at most
-256 (column, 4-cell segment) pairs, an interpolation region (tile + kernel
-width halo, with a 4-byte offset per row along the last axis) at most 64 KiB
-of LDS.  Results are cached in-process by a
+Only candidates the kernels accept are tried (``tile_ok``).  Results are cached in-process by a
 fingerprint of the trajectory (size, shape, coils, a strided subsample of the
 coordinates), and optionally in a JSON file (``HIPNUFFT_TUNE_CACHE``).
 
@@ -47,15 +44,21 @@ def _kernel_width(eps, upsampfac=2.0):
     return min(max(w, 2), 16)
 
 
-def tile_ok(tile, w, interp_group=1):
-    """Whether the kernels accept ``tile`` (per-axis, C order) at width w."""
+def tile_ok(tile, w, interp_group=1, interp_span=None):
+    """Whether the kernels accept ``tile`` (per-axis, C order) at width w:
+    at most 256 (column, 4-cell segment) pairs per spreading tile; an
+    interpolation region (interp_span tiles along the last axis, default 2
+    in 3D, plus the width, with a 4-byte offset per row) within 64 KiB for
+    several transforms per workgroup, within 160 KiB for one."""
     cells = int(np.prod(tile))
     zl = tile[-1]
     if (cells // zl) * -(-zl // 4) > 256:
         return False
-    region = [t + w for t in tile]
+    span = interp_span or (2 if len(tile) == 3 else 1)
+    region = [t + w for t in tile[:-1]] + [span * tile[-1] + w]
     rows = int(np.prod(region[:-1])) if len(tile) > 1 else 0   # row-offset table
-    return int(np.prod(region)) * 8 * max(interp_group, 1) + 4 * rows <= 65536
+    g = max(interp_group, 1)
+    return int(np.prod(region)) * 8 * g + 4 * rows <= (65536 if g > 1 else 163840)
 
 
 def tile_candidates(ndim, w):

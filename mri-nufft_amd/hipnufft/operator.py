@@ -75,7 +75,9 @@ class MRIHipNUFFT(FourierOperatorBase, _ToggleGradPlanMixin):
     B*C transforms per launch within a 24 GiB workspace budget);
     async_transfer; and kwargs eps (1e-6), upsampfac (2.0), gpu_device_id,
     tile, spread_group, interp_group, max_subproblem, spread_max,
-    spread_method ("auto", "cells", "blocks", "columns"), and autotune (time
+    spread_method ("auto", "cells", "blocks", "columns", "matrix"),
+    interp_span (bins along the last axis per interpolation work item), and
+    autotune (time
     candidate plan options on this trajectory and keep the fastest; explicit
     options win).
     """
@@ -101,7 +103,7 @@ class MRIHipNUFFT(FourierOperatorBase, _ToggleGradPlanMixin):
         # options given by the caller (not None / 0 / "auto") are kept over
         # the autotuner's choice
         explicit = {k for k in ("tile", "spread_group", "interp_group", "max_subproblem",
-                                "spread_max", "spread_method")
+                                "spread_max", "spread_method", "interp_span")
                     if kwargs.get(k) not in (None, 0, "auto")}
         self._plan_opts = dict(
             upsampfac=float(kwargs.pop("upsampfac", 2.0)),
@@ -112,6 +114,7 @@ class MRIHipNUFFT(FourierOperatorBase, _ToggleGradPlanMixin):
             max_subproblem=int(kwargs.pop("max_subproblem", 0)),
             spread_max=int(kwargs.pop("spread_max", 0)),
             spread_method=kwargs.pop("spread_method", 0),
+            interp_span=int(kwargs.pop("interp_span", 0)),
         )
         do_tune = bool(kwargs.pop("autotune", False))
         for k in ("gpu_method", "gpu_sort", "gpu_kerevalmeth", "gpu_spreadinterponly"):
