@@ -249,6 +249,37 @@ int hipnufft_gram_set_stream(hipnufft_gram* gram, void* stream);
 int hipnufft_gram_workspace_bytes(const hipnufft_gram* gram, size_t* bytes);
 int hipnufft_gram_destroy(hipnufft_gram* gram);
 
+/* ---- Stacked 2.5D (stack-of-stars) ---------------------------------------
+ * The Cartesian last image axis z (length Z) of images [.][X*Y][Z] against
+ * the NZ sampled planes of a 2D plan, laid out planes_d[t][i][X*Y] (the 2D
+ * plan's T*NZ transforms).  F is the centred DFT along z, F u[k] =
+ * sum_n u[n] exp(-2 pi i (n - Z/2)(k - Z/2) / Z) (integer Z/2; numpy's
+ * fftshift(fft(ifftshift(u)))); any Z up to 2730 (2,3,5-smooth lengths by
+ * Stockham passes, others by one direct DFT).  Global transform t0 + t is
+ * batch b = (t0 + t) / C, coil c = (t0 + t) % C.  Replaces the z-axis part
+ * of MRIStackedNUFFTGPU: _fftz / _ifftz (operators/stacked.py:467-489), the
+ * z_index plane selection of _op_sense_device / _op_calibless_device
+ * (:527-558, :639-662) and the zero-filled scatter + coil sum of
+ * _adj_op_sense_device / _adj_op_calibless_device (:711-740, :767-790). */
+typedef struct hipnufft_stack_s hipnufft_stack;
+/* z_index[NZ] (host): the sampled plane positions in [0, Z) (a repeated
+ * position is filled, in the adjoint, from its last plane, as the
+ * reference's fancy-index assignment does). */
+int hipnufft_stack_create(int64_t X, int64_t Y, int64_t Z, const int32_t* z_index, int NZ,
+                          void* stream, hipnufft_stack** out);
+/* planes_d[t][i] = scale * F(smaps_d[c] * img_d[b])[z_index[i]] for t < T
+ * (smaps_d NULL: img_d holds one image per global transform). */
+int hipnufft_stack_fwd(hipnufft_stack* st, const void* img_d, const void* smaps_d, int C,
+                       int t0, int T, float scale, void* planes_d);
+/* img_d[b] = scale * sum over the chunk's coils c of batch b of
+ *   conj(smaps_d[c]) * F^H(u), u[z_index[i]] = planes_d[t][i], 0 elsewhere;
+ * a batch whose coils started in an earlier chunk (t0 % C != 0) is added to.
+ * smaps_d NULL: img_d[t0 + t] = scale * F^H(u) per transform. */
+int hipnufft_stack_adj(hipnufft_stack* st, const void* planes_d, const void* smaps_d, int C,
+                       int t0, int T, float scale, void* img_d);
+int hipnufft_stack_set_stream(hipnufft_stack* st, void* stream);
+int hipnufft_stack_destroy(hipnufft_stack* st);
+
 /* Change the stream of subsequent calls. */
 int hipnufft_set_stream(hipnufft_plan* plan, void* stream);
 

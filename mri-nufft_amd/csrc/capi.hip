@@ -131,19 +131,27 @@ int lines_per_wg(int64_t n) {
   return L;
 }
 
-int make_axis(int64_t n, FFTAxis* ax, size_t* bytes) {
+// allow_direct: a length with other prime factors gets one direct DFT pass
+// (O(n^2) per line; the z axis of stacked transforms, whose length is the
+// image's)
+int make_axis(int64_t n, FFTAxis* ax, size_t* bytes, bool allow_direct = false) {
   ax->n = (int)n;
   ax->L = lines_per_wg(n);
   if ((size_t)n * (ax->L + 2) * sizeof(float2) > 65536 || n > kMaxElemsFFT)
     return set_error(HIPNUFFT_ERR_ARG, "fine grid axis too long for the LDS FFT (max 2730)");
   int64_t m = n;
   ax->nrad = 0;
+  ax->direct = 0;
   while (m % 8 == 0 && ax->nrad < kMaxRadices) { ax->rad[ax->nrad++] = 8; m /= 8; }
   if (m % 4 == 0) { ax->rad[ax->nrad++] = 4; m /= 4; }
   if (m % 2 == 0) { ax->rad[ax->nrad++] = 2; m /= 2; }
   while (m % 3 == 0 && ax->nrad < kMaxRadices) { ax->rad[ax->nrad++] = 3; m /= 3; }
   while (m % 5 == 0 && ax->nrad < kMaxRadices) { ax->rad[ax->nrad++] = 5; m /= 5; }
-  if (m != 1) return set_error(HIPNUFFT_ERR_ARG, "fine grid size is not 2,3,5-smooth");
+  if (m != 1) {
+    if (!allow_direct) return set_error(HIPNUFFT_ERR_ARG, "fine grid size is not 2,3,5-smooth");
+    ax->nrad = 0;
+    ax->direct = 1;
+  }
   for (int i = 0, ns = 1; i < ax->nrad; ns *= ax->rad[i], ++i)
     ax->ns_magic[i] = ns > 1 ? (uint32_t)((((uint64_t)1 << 32) + ns - 1) / ns) : 0u;
   std::vector<float2> tw(n);
@@ -410,6 +418,13 @@ struct hipnufft_gram_s {
   hipnufft_plan_s* fft = nullptr;
   float2* K = nullptr;
   size_t bytes = 0;
+};
+
+struct hipnufft_stack_s {
+  hn::StackGeom g;
+  int32_t* z_index = nullptr;  // device
+  int32_t* z_plane = nullptr;  // device
+  hipStream_t stream = nullptr;
 };
 
 using namespace hn;
@@ -894,6 +909,78 @@ int hipnufft_gram_set_stream(hipnufft_gram* g, void* stream) {
 int hipnufft_gram_workspace_bytes(const hipnufft_gram* g, size_t* bytes) {
   if (!g || !bytes) return set_error(HIPNUFFT_ERR_ARG, "null pointer");
   *bytes = g->bytes;
+  return 0;
+}
+
+int hipnufft_stack_create(int64_t X, int64_t Y, int64_t Z, const int32_t* z_index, int NZ,
+                          void* stream, hipnufft_stack** out) {
+  if (!out || (NZ > 0 && !z_index)) return set_error(HIPNUFFT_ERR_ARG, "null pointer");
+  *out = nullptr;
+  if (X < 1 || Y < 1 || Z < 1 || NZ < 0 || X * Y >= ((int64_t)1 << 31))
+    return set_error(HIPNUFFT_ERR_ARG, "invalid stack shape");
+  std::vector<int32_t> zp(Z, -1);
+  for (int i = 0; i < NZ; ++i) {
+    if (z_index[i] < 0 || z_index[i] >= Z) return set_error(HIPNUFFT_ERR_ARG, "z_index out of range");
+    zp[z_index[i]] = i;  // a repeated position keeps its last plane
+  }
+  auto* st = new hipnufft_stack_s();
+  st->stream = (hipStream_t)stream;
+  st->g.rows = X * Y;
+  st->g.Z = (int)Z;
+  st->g.NZ = NZ;
+  size_t bytes = 0;
+  int rc = make_axis(Z, &st->g.ax, &bytes, true);
+  if (!rc && NZ > 0 && hipMalloc((void**)&st->z_index, sizeof(int32_t) * NZ) != hipSuccess)
+    rc = set_error(HIPNUFFT_ERR_ALLOC, "stack: out of device memory");
+  if (!rc && hipMalloc((void**)&st->z_plane, sizeof(int32_t) * Z) != hipSuccess)
+    rc = set_error(HIPNUFFT_ERR_ALLOC, "stack: out of device memory");
+  if (!rc && NZ > 0 &&
+      hipMemcpy(st->z_index, z_index, sizeof(int32_t) * NZ, hipMemcpyHostToDevice) != hipSuccess)
+    rc = set_error(HIPNUFFT_ERR_HIP, "stack: copy failed");
+  if (!rc && hipMemcpy(st->z_plane, zp.data(), sizeof(int32_t) * Z, hipMemcpyHostToDevice) !=
+                 hipSuccess)
+    rc = set_error(HIPNUFFT_ERR_HIP, "stack: copy failed");
+  if (rc) {
+    hipnufft_stack_destroy(st);
+    return rc;
+  }
+  st->g.z_index = st->z_index;
+  st->g.z_plane = st->z_plane;
+  *out = st;
+  return 0;
+}
+
+int hipnufft_stack_fwd(hipnufft_stack* st, const void* img_d, const void* smaps_d, int C, int t0,
+                       int T, float scale, void* planes_d) {
+  if (!st || !img_d || !planes_d) return set_error(HIPNUFFT_ERR_ARG, "null pointer");
+  if (C < 1 || t0 < 0 || T < 0) return set_error(HIPNUFFT_ERR_ARG, "invalid chunk");
+  if (st->g.NZ == 0) return 0;
+  return stack_fwd(st->g, (const float2*)img_d, (const float2*)smaps_d, C, t0, T, scale,
+                   (float2*)planes_d, st->stream);
+}
+
+int hipnufft_stack_adj(hipnufft_stack* st, const void* planes_d, const void* smaps_d, int C,
+                       int t0, int T, float scale, void* img_d) {
+  if (!st || !img_d || (!planes_d && st->g.NZ > 0))
+    return set_error(HIPNUFFT_ERR_ARG, "null pointer");
+  if (C < 1 || t0 < 0 || T < 0) return set_error(HIPNUFFT_ERR_ARG, "invalid chunk");
+  return stack_adj(st->g, (const float2*)planes_d, (const float2*)smaps_d, C, t0, T, scale,
+                   (float2*)img_d, st->stream);
+}
+
+int hipnufft_stack_set_stream(hipnufft_stack* st, void* stream) {
+  if (!st) return set_error(HIPNUFFT_ERR_ARG, "null stack");
+  st->stream = (hipStream_t)stream;
+  return 0;
+}
+
+int hipnufft_stack_destroy(hipnufft_stack* st) {
+  if (!st) return 0;
+  (void)hipStreamSynchronize(st->stream);
+  (void)hipFree(st->g.ax.tw);
+  (void)hipFree(st->z_index);
+  (void)hipFree(st->z_plane);
+  delete st;
   return 0;
 }
 

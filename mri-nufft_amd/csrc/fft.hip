@@ -382,6 +382,185 @@ __global__ __launch_bounds__(kThreads) void k_rows_adj(RowsArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// stacked 2.5D: the Cartesian z axis (the image's last, contiguous axis)
+// ---------------------------------------------------------------------------
+
+// Direct DFT of the L lines of n elements in buf (any n): each output is a
+// sum over the whole line with twiddle index (j k) mod n advanced by k.
+template <int S, int EPT>
+__device__ void dft_lines_direct(float2* buf, const float2* twl, int n, int lgL, int LP) {
+  const int L = 1 << lgL, E = kThreads >> lgL;
+  const int l = threadIdx.x & (L - 1), e0 = threadIdx.x >> lgL;
+  float2 out[EPT];
+#pragma unroll
+  for (int q = 0; q < EPT; ++q) {
+    const int k = e0 + q * E;
+    float2 acc = make_float2(0.f, 0.f);
+    if (k < n) {
+      int t = 0;
+      for (int j = 0; j < n; ++j) {
+        float2 w = twl[t];
+        if (S > 0) w.y = -w.y;
+        const float2 v = buf[j * LP + l];
+        acc.x += v.x * w.x - v.y * w.y;
+        acc.y += v.x * w.y + v.y * w.x;
+        t += k;
+        if (t >= n) t -= n;
+      }
+    }
+    out[q] = acc;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < EPT; ++q) {
+    const int k = e0 + q * E;
+    if (k < n) buf[k * LP + l] = out[q];
+  }
+  __syncthreads();
+}
+
+template <int S, int EPT>
+__device__ __forceinline__ void fft_lines_any(float2* buf, const float2* twl, const FFTAxis& ax,
+                                              int lgL, int LP) {
+  if (ax.direct)
+    dft_lines_direct<S, EPT>(buf, twl, ax.n, lgL, LP);
+  else
+    fft_lines<S, EPT>(buf, twl, ax, lgL, LP);
+}
+
+struct StackArgs {
+  const float2* img;
+  const float2* smaps;
+  float2* planes;         // forward output
+  const float2* planes_in;  // adjoint input
+  float2* img_out;
+  const int32_t* z_index;
+  const int32_t* z_plane;
+  int C, t0, T;
+  int NZ;
+  int64_t rows;
+  float scale;
+  FFTAxis ax;
+  int lgL;
+};
+
+// rows r0 .. r0 + L of transform t: (smap *) image rows -> centred DFT along
+// z -> the NZ sampled positions, written plane-major (L adjacent rows of a
+// plane are one contiguous run)
+template <int EPT>
+__global__ __launch_bounds__(kThreads) void k_stack_fwd(StackArgs a) {
+  extern __shared__ float2 buf[];
+  const int L = 1 << a.lgL, LP = L + 1;
+  const float2* twl = stage_twiddles(buf, a.ax, LP);
+  const int Z = a.ax.n, h = Z / 2;
+  const int64_t r0 = (int64_t)blockIdx.x * L;
+  const int t = blockIdx.y, tt = a.t0 + t;
+  // line-major loads: thread (l, x0) holds positions x0 + j * TPL of row r0 + l
+  const int TPL = kThreads >> a.lgL;
+  {
+    const int l = threadIdx.x / TPL, x0 = threadIdx.x - l * TPL;
+    const int64_t r = r0 + l;
+    const bool rok = r < a.rows;
+    const int64_t N = a.rows * Z;
+    const float2* src = a.img + (int64_t)(a.smaps ? tt / a.C : tt) * N + r * Z;
+    const float2* sm = a.smaps ? a.smaps + (int64_t)(tt % a.C) * N + r * Z : nullptr;
+    float2 v[EPT], sv[EPT];
+#pragma unroll
+    for (int j = 0; j < EPT; ++j) {
+      const int x = x0 + j * TPL;
+      v[j] = make_float2(0.f, 0.f);
+      sv[j] = make_float2(1.f, 0.f);
+      if (rok && x < Z) {
+        v[j] = src[x];
+        if (sm) sv[j] = sm[x];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < EPT; ++j) {
+      const int x = x0 + j * TPL;
+      // centred input: position x sits at (x - Z/2) mod Z of the DFT line
+      if (x < Z) buf[(x < h ? x - h + Z : x - h) * LP + l] = cmul(v[j], sv[j]);
+    }
+  }
+  __syncthreads();
+  fft_lines_any<-1, EPT>(buf, twl, a.ax, a.lgL, LP);
+  // plane-major stores: thread (l, i0) writes planes i0 + q * (kThreads / L)
+  const int l = threadIdx.x & (L - 1), i0 = threadIdx.x >> a.lgL;
+  const int64_t r = r0 + l;
+  if (r >= a.rows) return;
+  float2* dst = a.planes + (int64_t)t * a.NZ * a.rows + r;
+  for (int i = i0; i < a.NZ; i += TPL) {
+    const int k = a.z_index[i];  // centred output k is DFT index (k - Z/2) mod Z
+    dst[(int64_t)i * a.rows] = cscale(buf[(k < h ? k - h + Z : k - h) * LP + l], a.scale);
+  }
+}
+
+// rows r0 .. r0 + L of the output image(s) of block y: per transform of the
+// chunk, zero-filled z line from the planes -> centred inverse DFT -> conj
+// smap, summed over the transforms of one image in registers
+template <int EPT>
+__global__ __launch_bounds__(kThreads) void k_stack_adj(StackArgs a) {
+  extern __shared__ float2 buf[];
+  const int L = 1 << a.lgL, LP = L + 1;
+  const float2* twl = stage_twiddles(buf, a.ax, LP);
+  const int Z = a.ax.n, h = Z / 2;
+  const int64_t r0 = (int64_t)blockIdx.x * L;
+  int tb, te, out_img;
+  if (a.smaps) {
+    const int b = a.t0 / a.C + blockIdx.y;
+    out_img = b;
+    tb = max(a.t0, b * a.C);
+    te = min(a.t0 + a.T, (b + 1) * a.C);
+  } else {
+    out_img = a.t0 + blockIdx.y;
+    tb = out_img;
+    te = out_img + 1;
+  }
+  const bool add_existing = a.smaps && (tb % a.C != 0);
+  const int TPL = kThreads >> a.lgL;
+  const int lq = threadIdx.x & (L - 1), q0 = threadIdx.x >> a.lgL;  // gather mapping
+  const int lr = threadIdx.x / TPL, x0 = threadIdx.x - lr * TPL;    // row mapping
+  const int64_t rq = r0 + lq, rr = r0 + lr;
+  const int64_t N = a.rows * Z;
+  float2 acc[EPT];
+#pragma unroll
+  for (int j = 0; j < EPT; ++j) acc[j] = make_float2(0.f, 0.f);
+  for (int tt = tb; tt < te; ++tt) {
+    const float2* pl = a.planes_in + (int64_t)(tt - a.t0) * a.NZ * a.rows + rq;
+    // DFT position j holds centred position (j + Z/2) mod Z
+    for (int j = q0; j < Z; j += TPL) {
+      const int k = j + h < Z ? j + h : j + h - Z;
+      const int i = a.z_plane[k];
+      buf[j * LP + lq] = (i >= 0 && rq < a.rows) ? pl[(int64_t)i * a.rows] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    fft_lines_any<1, EPT>(buf, twl, a.ax, a.lgL, LP);
+    const float2* sm = a.smaps ? a.smaps + (int64_t)(tt % a.C) * N + rr * Z : nullptr;
+#pragma unroll
+    for (int j = 0; j < EPT; ++j) {
+      const int x = x0 + j * TPL;
+      if (x < Z && rr < a.rows) {
+        float2 v = cscale(buf[(x < h ? x - h + Z : x - h) * LP + lr], a.scale);
+        if (sm) {
+          float2 sv = sm[x];
+          sv.y = -sv.y;
+          v = cmul(v, sv);
+        }
+        acc[j] = cadd(acc[j], v);
+      }
+    }
+    __syncthreads();
+  }
+  if (rr >= a.rows) return;
+  float2* dst = a.img_out + (int64_t)out_img * N + rr * Z;
+#pragma unroll
+  for (int j = 0; j < EPT; ++j) {
+    const int x = x0 + j * TPL;
+    if (x < Z) dst[x] = add_existing ? cadd(acc[j], dst[x]) : acc[j];
+  }
+}
+
+// ---------------------------------------------------------------------------
 // strided axes (0 and 1), in place
 // ---------------------------------------------------------------------------
 struct StridedArgs {
@@ -702,6 +881,55 @@ int gram_chain(const hipnufft_plan_s* p, const float2* img, const float2* smaps,
       return rc;
   }
   return rows_adj(p, out, smaps, C, t0, T, 1.0f, 1, 0);
+}
+
+int stack_fwd(const StackGeom& g, const float2* img, const float2* smaps, int C, int t0, int T,
+              float scale, float2* planes, hipStream_t s) {
+  if (T <= 0 || g.rows <= 0) return 0;
+  StackArgs a{};
+  a.img = img;
+  a.smaps = smaps;
+  a.planes = planes;
+  a.z_index = g.z_index;
+  a.C = C;
+  a.t0 = t0;
+  a.T = T;
+  a.NZ = g.NZ;
+  a.rows = g.rows;
+  a.scale = scale;
+  a.ax = g.ax;
+  a.lgL = lg2(g.ax.L);
+  const dim3 grd((unsigned)((g.rows + g.ax.L - 1) / g.ax.L), (unsigned)T);
+  HN_EPT_SWITCH(ept_of(g.ax), {
+    hipLaunchKernelGGL((k_stack_fwd<EE>), grd, dim3(kThreads), fft_lds(g.ax), s, a);
+  });
+  HN_LAUNCH_CHECK();
+  return 0;
+}
+
+int stack_adj(const StackGeom& g, const float2* planes, const float2* smaps, int C, int t0, int T,
+              float scale, float2* img, hipStream_t s) {
+  if (T <= 0 || g.rows <= 0) return 0;
+  StackArgs a{};
+  a.planes_in = planes;
+  a.smaps = smaps;
+  a.img_out = img;
+  a.z_plane = g.z_plane;
+  a.C = C;
+  a.t0 = t0;
+  a.T = T;
+  a.NZ = g.NZ;
+  a.rows = g.rows;
+  a.scale = scale;
+  a.ax = g.ax;
+  a.lgL = lg2(g.ax.L);
+  const int nimg = smaps ? ((t0 + T - 1) / C - t0 / C + 1) : T;
+  const dim3 grd((unsigned)((g.rows + g.ax.L - 1) / g.ax.L), (unsigned)nimg);
+  HN_EPT_SWITCH(ept_of(g.ax), {
+    hipLaunchKernelGGL((k_stack_adj<EE>), grd, dim3(kThreads), fft_lds(g.ax), s, a);
+  });
+  HN_LAUNCH_CHECK();
+  return 0;
 }
 
 }  // namespace hn

@@ -42,6 +42,7 @@ struct FFTAxis {
   int rad[kMaxRadices] = {0};
   uint32_t ns_magic[kMaxRadices] = {0};  // ceil(2^32 / Ns) of each pass (Ns > 1)
   float2* tw = nullptr;  // tw[t] = exp(-2 pi i t / n), t < n
+  int direct = 0;        // n is not 2,3,5-smooth: one direct DFT pass (z axis of stacks)
 };
 
 // Geometry of the points / tiles for spreading and interpolation.  Arrays
@@ -290,6 +291,25 @@ void free_plan(hipnufft_plan_s* p);
 
 int dot_c64(const float2* a, const float2* b, int64_t rows, int64_t n, int conj,
             double* out, double* ws, hipStream_t s);
+
+// Stacked 2.5D: the Cartesian last image axis (z, length Z = ax.n) of
+// images [.][rows][Z] against the sampled planes [t][NZ][rows] of a 2D plan.
+// Global transform t0 + t = b * C + c: image b, map c (smaps NULL: image
+// t0 + t).  Forward: planes[t][i][r] = scale * F(z_index[i]) with F the
+// centred unitary-free DFT along z of smaps[c] * img[b] (row r).  Adjoint:
+// img[b] (+)= scale * sum_c conj(smaps[c]) * F^H(zero-filled planes), plane
+// z_plane[k] (or none, -1) at position k.
+struct StackGeom {
+  int64_t rows = 0;
+  int Z = 0, NZ = 0;
+  const int32_t* z_index = nullptr;  // [NZ] device
+  const int32_t* z_plane = nullptr;  // [Z] device
+  FFTAxis ax;
+};
+int stack_fwd(const StackGeom& g, const float2* img, const float2* smaps, int C, int t0, int T,
+              float scale, float2* planes, hipStream_t s);
+int stack_adj(const StackGeom& g, const float2* planes, const float2* smaps, int C, int t0, int T,
+              float scale, float2* img, hipStream_t s);
 
 // Exclusive scan of n int32 (device), result in out (may alias in), total in
 // *total (host, synchronises the stream) if total != nullptr.

@@ -13,10 +13,13 @@ from .base import FourierOperatorBase, check_backend, get_operator, list_backend
 from .density import get_density, pipe
 from .operator import MRIHipNUFFT
 from .optim import cg, get_optimizer, lsmr, lsqr
+from .stacked import MRIStackedHipNUFFT, MRIStackedNUFFT, stacked2traj3d, traj3d2stacked
 
 __all__ = [
     "FourierOperatorBase",
     "MRIHipNUFFT",
+    "MRIStackedHipNUFFT",
+    "MRIStackedNUFFT",
     "cg",
     "check_backend",
     "get_density",
@@ -28,6 +31,8 @@ __all__ = [
     "pipe",
     "power_method",
     "register_into_mrinufft",
+    "stacked2traj3d",
+    "traj3d2stacked",
 ]
 
 

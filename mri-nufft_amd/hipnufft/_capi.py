@@ -11,6 +11,8 @@ from __future__ import annotations
 import ctypes
 import os
 import threading
+
+import numpy as np
 from pathlib import Path
 
 import torch  # noqa: F401  (must be imported before the library is loaded)
@@ -85,6 +87,12 @@ _SIGNATURES = {
     "hipnufft_gram_set_stream": [_P, _P],
     "hipnufft_gram_workspace_bytes": [_P, ctypes.POINTER(ctypes.c_size_t)],
     "hipnufft_gram_destroy": [_P],
+    "hipnufft_stack_create": [_I64, _I64, _I64, ctypes.POINTER(ctypes.c_int32), _I, _P,
+                              ctypes.POINTER(_P)],
+    "hipnufft_stack_fwd": [_P, _P, _P, _I, _I, _I, _F, _P],
+    "hipnufft_stack_adj": [_P, _P, _P, _I, _I, _I, _F, _P],
+    "hipnufft_stack_set_stream": [_P, _P],
+    "hipnufft_stack_destroy": [_P],
 }
 
 EXPORTED = sorted(list(_SIGNATURES) + ["hipnufft_last_error"])
@@ -301,6 +309,51 @@ class RawPlan:
         h, self._h = getattr(self, "_h", None), None
         if h and _lib is not None:
             _lib.hipnufft_plan_destroy(h)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class RawStack:
+    """Owns one ``hipnufft_stack``: the z-axis passes of a stacked 2.5D
+    transform between images [.][X][Y][Z] and planes [T][NZ][X][Y]."""
+
+    def __init__(self, shape, z_index, stream=None):
+        L = lib()
+        X, Y, Z = (int(s) for s in shape)
+        self.shape = (X, Y, Z)
+        zi = np.ascontiguousarray(np.asarray(z_index, dtype=np.int32).reshape(-1))
+        self.n_planes = int(zi.size)
+        self.stream = current_stream() if stream is None else stream
+        h = _P()
+        check(L.hipnufft_stack_create(X, Y, Z, zi.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                      self.n_planes, self.stream, ctypes.byref(h)),
+              "stack_create")
+        self._h = h
+
+    def _sync_stream(self):
+        s = current_stream()
+        if s != self.stream:
+            self.stream = s
+            check(lib().hipnufft_stack_set_stream(self._h, s))
+
+    def fwd(self, img, smaps, planes, C, t0, T, scale):
+        self._sync_stream()
+        check(lib().hipnufft_stack_fwd(self._h, ptr(img), ptr(smaps), int(C), int(t0), int(T),
+                                       float(scale), ptr(planes)), "stack_fwd")
+
+    def adj(self, planes, smaps, img, C, t0, T, scale):
+        self._sync_stream()
+        check(lib().hipnufft_stack_adj(self._h, ptr(planes), ptr(smaps), int(C), int(t0),
+                                       int(T), float(scale), ptr(img)), "stack_adj")
+
+    def close(self):
+        h, self._h = getattr(self, "_h", None), None
+        if h and _lib is not None:
+            _lib.hipnufft_stack_destroy(h)
 
     def __del__(self):
         try:

@@ -1,0 +1,152 @@
+"""Stacked 2.5D NUFFT on the GPU.
+
+Mirrors tests/operators/test_stacked_gpu.py of the reference (batch / coil /
+SENSE combinations, "full" and "random_mask" z indices, forward, adjoint,
+adjointness, operator reuse, trajectory conversion) with tighter bars:
+
+* forward and adjoint against the FP64 stacked oracle (oracle/stacked.py,
+  itself pinned to the 3D NDFT), rel. L2 <= 1e-5 (NUFFT tolerance 1e-6);
+* for an even number of planes, against the full 3D operator on the
+  equivalent 3D trajectory, rel. L2 <= 1e-5 (the reference holds
+  rtol 1e-1 / atol 1e-4 here);
+* z lengths 20 (2,3,5-smooth: Stockham passes), 22 and 21 (other primes:
+  the direct DFT pass, even and odd).
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from helpers import gauss_smaps, radial2d, rand_cplx, rel_l2
+from oracle import stacked as ost
+
+pytestmark = pytest.mark.gpu
+
+CASES = [(1, 1, False), (1, 4, False), (1, 4, True), (3, 4, False), (3, 4, True)]
+
+
+def _setup(B, C, sense, zmode, Z, seed=0, n_trans="ref", density=False):
+    from hipnufft import get_operator
+
+    rng = np.random.default_rng(seed)
+    s2 = radial2d(12, 40).reshape(-1, 2).astype(np.float32)
+    shape = (24, 24, Z)
+    if zmode == "full":
+        z_index = np.arange(Z)
+    else:
+        z_index = rng.random(Z) > 0.5
+    zi = np.arange(Z)[z_index]
+    smaps = gauss_smaps(C, shape, seed) if sense else None
+    nt = (2 if C > 1 else 1) if n_trans == "ref" else n_trans
+    op = get_operator("stacked-hipnufft")(s2, shape, smaps=smaps, n_coils=C, n_batchs=B,
+                                          z_index=z_index, n_trans=nt, density=density)
+    return op, s2, zi, shape, smaps
+
+
+def _img(op, seed):
+    B, C = op.n_batchs, op.n_coils
+    return rand_cplx((B, *op.shape) if op.uses_sense else (B, C, *op.shape), seed)
+
+
+@pytest.mark.parametrize("Z", [20, 22, 21])
+@pytest.mark.parametrize("zmode", ["full", "random_mask"])
+@pytest.mark.parametrize("B,C,sense", CASES)
+def test_stack_forward_adjoint_vs_oracle(B, C, sense, zmode, Z):
+    op, s2, zi, shape, smaps = _setup(B, C, sense, zmode, Z)
+    x = _img(op, 1)
+    y = op.op(x)
+    yref = ost.op(s2, zi, x, shape, smaps)
+    assert rel_l2(np.asarray(y).reshape(yref.shape), yref) < 1e-5
+    ksp = rand_cplx((B, C, op.n_samples), 2)
+    z = op.adj_op(ksp)
+    zref = ost.adj(s2, zi, ksp, shape, smaps, n_batchs=B)
+    assert rel_l2(np.asarray(z).reshape(zref.shape), zref) < 1e-5
+    # adjointness <A x, y> = <x, A^H y>
+    lhs = np.vdot(np.asarray(y).ravel().astype(np.complex128), ksp.ravel())
+    rhs = np.vdot(x.ravel().astype(np.complex128), np.asarray(z).ravel())
+    assert abs(lhs - rhs) <= 1e-5 * abs(lhs)
+
+
+@pytest.mark.parametrize("B,C,sense", [(1, 4, True), (3, 4, False)])
+def test_stack_equals_full_3d_operator(B, C, sense):
+    from hipnufft import MRIHipNUFFT
+    from hipnufft.stacked import stacked2traj3d
+
+    op, s2, zi, shape, smaps = _setup(B, C, sense, "random_mask", 20, seed=3)
+    ref = MRIHipNUFFT(stacked2traj3d(s2, zi, shape[2]), shape, n_coils=C, n_batchs=B,
+                      smaps=smaps)
+    x = torch.from_numpy(_img(op, 4)).cuda()
+    y, yr = op.op(x), ref.op(x)
+    assert rel_l2(y.cpu().numpy(), yr.cpu().numpy()) < 1e-5
+    ksp = torch.from_numpy(rand_cplx((B, C, op.n_samples), 5)).cuda()
+    assert rel_l2(op.adj_op(ksp).cpu().numpy(), ref.adj_op(ksp).cpu().numpy()) < 1e-5
+
+
+def test_stack_chunking_and_density():
+    """One (batch, coil) pair per launch equals all pairs in one launch; the
+    Pipe density of the 2D pattern weighs the adjoint like the oracle's."""
+    a, s2, zi, shape, smaps = _setup(2, 3, True, "random_mask", 22, seed=6, n_trans=1,
+                                     density="pipe")
+    b, *_ = _setup(2, 3, True, "random_mask", 22, seed=6, n_trans=None, density="pipe")
+    assert b.n_trans == 6 and a.n_trans == 1
+    d = a.density.cpu().numpy()
+    np.testing.assert_allclose(d, b.density.cpu().numpy(), rtol=1e-6)
+    ksp = rand_cplx((2, 3, a.n_samples), 7)
+    za, zb = a.adj_op(ksp), b.adj_op(ksp)
+    assert rel_l2(za, zb) < 1e-6
+    zref = ost.adj(s2, zi, ksp, shape, smaps, density=d, n_batchs=2)
+    assert rel_l2(np.asarray(za).reshape(zref.shape), zref) < 1e-5
+    x = _img(a, 8)
+    assert rel_l2(a.op(x), b.op(x)) < 1e-6
+
+
+def test_stack_repeated_plane_and_lipschitz():
+    """A repeated z position is filled from its last plane in the adjoint (the
+    reference's fancy-index assignment); the Lipschitz constant is the 2D
+    single-coil operator's with the density, as the reference returns it."""
+    from hipnufft import MRIHipNUFFT, MRIStackedHipNUFFT
+
+    s2 = radial2d(10, 32).reshape(-1, 2).astype(np.float32)
+    shape = (16, 16, 12)
+    zi = np.array([3, 7, 3, 10])
+    op = MRIStackedHipNUFFT(s2, shape, n_coils=2, z_index=zi, n_trans=2)
+    ksp = rand_cplx((1, 2, op.n_samples), 9)
+    z = op.adj_op(ksp)
+    zref = ost.adj(s2, zi, ksp, shape, n_batchs=1)
+    assert rel_l2(np.asarray(z).reshape(zref.shape), zref) < 1e-5
+    x = rand_cplx((1, 2, *shape), 10)
+    assert rel_l2(np.asarray(op.op(x)).reshape(1, 2, -1), ost.op(s2, zi, x, shape)) < 1e-5
+    L = op.get_lipschitz_cst(max_iter=20, seed=0)
+    L2 = MRIHipNUFFT(s2, shape[:2], n_coils=1, squeeze_dims=True).get_lipschitz_cst(
+        max_iter=20, seed=0)
+    assert abs(float(L) - float(L2)) <= 1e-6 * abs(float(L2))
+
+
+def test_stack_reuse_backend_operator():
+    from hipnufft import MRIStackedNUFFT
+
+    op, s2, zi, shape, smaps = _setup(1, 4, True, "random_mask", 20, seed=11)
+    reuse = MRIStackedNUFFT(backend=op.operator, shape=op.shape, samples=op.samples,
+                            z_index=op.z_index, n_coils=op.n_coils, n_batchs=op.n_batchs,
+                            smaps=smaps)
+    assert reuse.operator is op.operator
+    x = _img(op, 12)
+    np.testing.assert_array_equal(np.asarray(reuse.op(x)), np.asarray(op.op(x)))
+
+
+def test_stack_cg_reconstruction():
+    """CG on the stacked operator recovers the image the full 3D operator
+    recovers from the same data (density off, same Lipschitz constant)."""
+    from hipnufft import MRIHipNUFFT, cg
+    from hipnufft.stacked import stacked2traj3d
+
+    op, s2, zi, shape, smaps = _setup(1, 4, True, "full", 20, seed=13)
+    x = torch.from_numpy(_img(op, 14)).cuda()
+    y = op.op(x)
+    ref = MRIHipNUFFT(stacked2traj3d(s2, zi, shape[2]), shape, n_coils=4, smaps=smaps)
+    L = float(op.get_lipschitz_cst(max_iter=10, seed=0))
+    op.get_lipschitz_cst = lambda **_: L
+    ref.get_lipschitz_cst = lambda **_: L
+    a = cg(op, y, max_iter=8)
+    b = cg(ref, y.reshape(1, 4, -1), max_iter=8)
+    assert rel_l2(a.cpu().numpy().ravel(), b.cpu().numpy().ravel()) < 1e-4
