@@ -280,6 +280,35 @@ int hipnufft_stack_adj(hipnufft_stack* st, const void* planes_d, const void* sma
 int hipnufft_stack_set_stream(hipnufft_stack* st, void* stream);
 int hipnufft_stack_destroy(hipnufft_stack* st);
 
+/* ---- Coil-map estimation (low-frequency method) ------------------------------
+ * The array passes of extras/smaps.py low_frequency (:219-306) around its
+ * least-squares NUFFT solve.  Windows of the k-space centre
+ * (_extract_kspace_center, :83-170) for samples traj_d[M][d] in cycles
+ * ([-0.5, 0.5)) and thresholds threshold[d] (host):
+ *   ELLIPSE  1 if sum_d (k_d / t_d)^2 <= 1, else 0;
+ *   RECT     1 if |k_d| <= t_d on every axis, else 0 (the reference keeps
+ *            only those samples: use the weights to select them);
+ *   HANN     0.5 + 0.5 cos(pi |k| / t_0);  HAMMING with a0 = 0.53836.
+ * ksp_out_d[c][m] = w_m * ksp_in_d[c][m] for C coils (ksp_in_d NULL: none),
+ * weights_d[m] = w_m (NULL: not written). */
+#define HIPNUFFT_WINDOW_ELLIPSE 0
+#define HIPNUFFT_WINDOW_RECT 1
+#define HIPNUFFT_WINDOW_HANN 2
+#define HIPNUFFT_WINDOW_HAMMING 3
+int hipnufft_kspace_window(const float* traj_d, int64_t M, int d, const float* threshold,
+                           int kind, const void* ksp_in_d, void* ksp_out_d, int C,
+                           float* weights_d, void* stream);
+/* maps_d[c][n] /= sqrt(sum_c |maps_d[c][n]|^2) + eps (the SOS division,
+ * smaps.py:300-306; the sum in FP64). */
+int hipnufft_rss_normalize(void* maps_d, int C, int64_t N, float eps, void* stream);
+/* |maps_d[c]| filtered by a Gaussian of sigma[q] cells along image axis q
+ * (0: none), radius int(truncate * sigma + 0.5), "nearest" edges, the phase
+ * kept -- skimage.filters.gaussian(|maps|, sigma=(0,) + sigma) * exp(i
+ * angle(maps)) (smaps.py:293-298); a zero map value takes phase 0 (numpy's
+ * angle of a signed zero is 0 or pi).  work_d: 2 * C * prod(shape) floats. */
+int hipnufft_blur_magnitude(void* maps_d, int C, int ndim, const int64_t* shape,
+                            const double* sigma, double truncate, float* work_d, void* stream);
+
 /* Change the stream of subsequent calls. */
 int hipnufft_set_stream(hipnufft_plan* plan, void* stream);
 

@@ -984,6 +984,37 @@ int hipnufft_stack_destroy(hipnufft_stack* st) {
   return 0;
 }
 
+int hipnufft_kspace_window(const float* traj_d, int64_t M, int d, const float* threshold,
+                           int kind, const void* ksp_in_d, void* ksp_out_d, int C,
+                           float* weights_d, void* stream) {
+  if (!traj_d || !threshold || (ksp_in_d && !ksp_out_d))
+    return set_error(HIPNUFFT_ERR_ARG, "null pointer");
+  if (M < 0 || d < 1 || d > 3 || C < 0 || kind < HIPNUFFT_WINDOW_ELLIPSE ||
+      kind > HIPNUFFT_WINDOW_HAMMING)
+    return set_error(HIPNUFFT_ERR_ARG, "invalid window arguments");
+  for (int q = 0; q < d; ++q)
+    if (!(threshold[q] > 0.0f)) return set_error(HIPNUFFT_ERR_ARG, "window threshold must be > 0");
+  return kspace_window(traj_d, M, d, threshold, kind, (const float2*)ksp_in_d,
+                       (float2*)ksp_out_d, C, weights_d, (hipStream_t)stream);
+}
+
+int hipnufft_rss_normalize(void* maps_d, int C, int64_t N, float eps, void* stream) {
+  if (!maps_d) return set_error(HIPNUFFT_ERR_ARG, "null pointer");
+  if (C < 1 || N < 0) return set_error(HIPNUFFT_ERR_ARG, "invalid map shape");
+  return rss_normalize((float2*)maps_d, C, N, eps, (hipStream_t)stream);
+}
+
+int hipnufft_blur_magnitude(void* maps_d, int C, int ndim, const int64_t* shape,
+                            const double* sigma, double truncate, float* work_d, void* stream) {
+  if (!maps_d || !shape || !sigma || !work_d) return set_error(HIPNUFFT_ERR_ARG, "null pointer");
+  if (C < 1 || ndim < 1 || ndim > 3 || !(truncate > 0.0))
+    return set_error(HIPNUFFT_ERR_ARG, "invalid blur arguments");
+  for (int q = 0; q < ndim; ++q)
+    if (shape[q] < 1 || sigma[q] < 0.0) return set_error(HIPNUFFT_ERR_ARG, "invalid blur arguments");
+  return blur_magnitude((float2*)maps_d, C, ndim, shape, sigma, truncate, work_d,
+                        (hipStream_t)stream);
+}
+
 int hipnufft_gram_destroy(hipnufft_gram* g) {
   if (!g) return 0;
   if (g->fft) {

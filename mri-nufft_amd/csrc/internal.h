@@ -311,6 +311,20 @@ int stack_fwd(const StackGeom& g, const float2* img, const float2* smaps, int C,
 int stack_adj(const StackGeom& g, const float2* planes, const float2* smaps, int C, int t0, int T,
               float scale, float2* img, hipStream_t s);
 
+// ---- maps.hip ---------------------------------------------------------------
+// Coil-map estimation helpers (values of the HIPNUFFT_WINDOW_* constants).
+enum { kWindowEllipse = 0, kWindowRect = 1, kWindowHann = 2, kWindowHamming = 3 };
+// out[c][m] = w_m * in[c][m] (in may be NULL: weights only), weights[m] = w_m
+// (may be NULL) for the window of traj[M][d] (cycles) with thresholds thr[d]
+int kspace_window(const float* traj, int64_t M, int d, const float* thr, int kind,
+                  const float2* in, float2* out, int C, float* weights, hipStream_t s);
+// maps[c][n] /= sqrt(sum_c |maps[c][n]|^2) + eps
+int rss_normalize(float2* maps, int C, int64_t N, float eps, hipStream_t s);
+// |maps| blurred by a separable Gaussian per axis (sigma[q], scipy's
+// truncation and "nearest" edges), phase kept; work holds 2 C N floats
+int blur_magnitude(float2* maps, int C, int ndim, const int64_t* shape, const double* sigma,
+                   double truncate, float* work, hipStream_t s);
+
 // Exclusive scan of n int32 (device), result in out (may alias in), total in
 // *total (host, synchronises the stream) if total != nullptr.
 int scan_exclusive(const int32_t* in, int32_t* out, int64_t n,

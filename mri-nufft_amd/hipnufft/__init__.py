@@ -13,6 +13,7 @@ from .base import FourierOperatorBase, check_backend, get_operator, list_backend
 from .density import get_density, pipe
 from .operator import MRIHipNUFFT
 from .optim import cg, get_optimizer, lsmr, lsqr
+from .smaps import get_smaps, low_frequency
 from .stacked import MRIStackedHipNUFFT, MRIStackedNUFFT, stacked2traj3d, traj3d2stacked
 
 __all__ = [
@@ -25,6 +26,8 @@ __all__ = [
     "get_density",
     "get_operator",
     "get_optimizer",
+    "get_smaps",
+    "low_frequency",
     "list_backends",
     "lsmr",
     "lsqr",

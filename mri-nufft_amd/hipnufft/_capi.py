@@ -93,6 +93,10 @@ _SIGNATURES = {
     "hipnufft_stack_adj": [_P, _P, _P, _I, _I, _I, _F, _P],
     "hipnufft_stack_set_stream": [_P, _P],
     "hipnufft_stack_destroy": [_P],
+    "hipnufft_kspace_window": [_P, _I64, _I, ctypes.POINTER(_F), _I, _P, _P, _I, _P, _P],
+    "hipnufft_rss_normalize": [_P, _I, _I64, _F, _P],
+    "hipnufft_blur_magnitude": [_P, _I, _I, ctypes.POINTER(_I64), ctypes.POINTER(ctypes.c_double),
+                                ctypes.c_double, _P, _P],
 }
 
 EXPORTED = sorted(list(_SIGNATURES) + ["hipnufft_last_error"])
@@ -315,6 +319,47 @@ class RawPlan:
             self.close()
         except Exception:
             pass
+
+
+WINDOWS = {"ellipse": 0, "rect": 1, "hann": 2, "hanning": 2, "hamming": 3}
+
+
+def kspace_window(traj, threshold, kind, ksp=None, weights=True):
+    """Window of the k-space centre on the device: (weights (M,) or None,
+    windowed copy of ksp (C, M) or None).  traj (M, d) float32 in cycles."""
+    t = traj.to(torch.float32).contiguous()
+    M, d = t.shape
+    thr = (_F * d)(*[float(x) for x in threshold])
+    w = torch.empty(M, dtype=torch.float32, device=t.device) if weights else None
+    k = out = None
+    C = 0
+    if ksp is not None:
+        k = ksp.reshape(-1, M).contiguous()
+        C = k.shape[0]
+        out = torch.empty_like(k)
+    check(lib().hipnufft_kspace_window(ptr(t), M, d, thr, WINDOWS[kind], ptr(k), ptr(out), C,
+                                       ptr(w), current_stream()), "window")
+    return w, out
+
+
+def rss_normalize(maps, eps=0.0):
+    """maps (C, *shape) complex64 device tensor, divided in place by its
+    root sum of squares over coils (+ eps)."""
+    C = maps.shape[0]
+    check(lib().hipnufft_rss_normalize(ptr(maps), C, maps[0].numel(), float(eps),
+                                       current_stream()), "rss_normalize")
+    return maps
+
+
+def blur_magnitude(maps, sigma, truncate=4.0):
+    """Gaussian blur of |maps| per image axis (sigma per axis), phase kept, in place."""
+    C, shape = maps.shape[0], tuple(maps.shape[1:])
+    work = torch.empty(2 * maps.numel(), dtype=torch.float32, device=maps.device)
+    sh = (_I64 * len(shape))(*shape)
+    sg = (ctypes.c_double * len(shape))(*[float(x) for x in sigma])
+    check(lib().hipnufft_blur_magnitude(ptr(maps), C, len(shape), sh, sg, float(truncate),
+                                        ptr(work), current_stream()), "blur_magnitude")
+    return maps
 
 
 class RawStack:
