@@ -147,6 +147,29 @@ int hipnufft_sense_adj(hipnufft_plan* plan, const void* ksp_d,
                        const void* smaps_d, const float* density_d,
                        void* img_d, int B, int C, float scale);
 
+/* Off-resonance corrected SENSE / multi-coil forward with L interpolators:
+ *   ksp_d[b][c][m] = sum_l bt_d[m % NK][l] * scale * F(smaps_d[c] * cmap_d[l] * img_d[b])[m]
+ * (img_d[b][c] instead of smaps_d[c] * img_d[b] when smaps_d == NULL).
+ * cmap_d: the spatial interpolators C, complex64 [L][N]; bt_d: the temporal
+ * interpolators B, complex64 [NK][L] (NK samples per shot, NK divides M).
+ * The B*C*L transforms go through one batched chain (chunked by the plan's
+ * ntrans); uses the plan's ntrans*M residual workspace.  Needs a plan with
+ * one point set.  Replaces MRIFourierCorrected.op
+ * (operators/off_resonance.py:230-262). */
+int hipnufft_orc_op(hipnufft_plan* plan, const void* img_d, const void* smaps_d,
+                    const void* cmap_d, const void* bt_d, int L, int64_t NK,
+                    void* ksp_d, int B, int C, float scale);
+
+/* Off-resonance corrected adjoint:
+ *   img_d[b] = scale * sum_l conj(cmap_d[l]) * sum_c conj(smaps_d[c])
+ *              * F^H(density_d * conj(bt_d[. % NK][l]) * ksp_d[b][c])
+ * (img_d[b][c] without smaps; density_d may be NULL).  Replaces
+ * MRIFourierCorrected.adj_op (operators/off_resonance.py:264-302). */
+int hipnufft_orc_adj(hipnufft_plan* plan, const void* ksp_d, const void* smaps_d,
+                     const void* cmap_d, const void* bt_d, int L, int64_t NK,
+                     const float* density_d, void* img_d, int B, int C,
+                     float scale);
+
 /* Data consistency  grad = scale * A^H D (scale * A x - obs) with the same
  * SENSE / calibrationless conventions (D = diag(density_d), may be NULL); the
  * residual lives in a plan-owned ntrans*M buffer.  Replaces

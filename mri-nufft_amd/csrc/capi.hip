@@ -673,6 +673,78 @@ int hipnufft_sense_dc(hipnufft_plan* p, const void* img, const void* smaps, cons
   return 0;
 }
 
+}  // extern "C"
+
+namespace {
+
+// the checks and chunks shared by the two off-resonance entry points: the
+// transforms are [image][coil][l], per_img = (smaps ? C : 1) * L of them per
+// output image, chunks aligned to whole images when the workspace holds one
+int orc_setup(hipnufft_plan_s* p, const void* cmap, const void* bt, int L, int64_t NK, int B,
+              int C, bool sense, std::vector<Chunk>& chunks) {
+  int rc = need_pts(p);
+  if (rc) return rc;
+  if (p->sio) return set_error(HIPNUFFT_ERR_ARG, "the off-resonance operator needs a full NUFFT plan");
+  if (p->nsets != 1)
+    return set_error(HIPNUFFT_ERR_ARG, "the off-resonance operator needs a plan with one point set");
+  if (B < 1 || C < 1) return set_error(HIPNUFFT_ERR_ARG, "B and C must be >= 1");
+  if (L < 1 || NK < 1) return set_error(HIPNUFFT_ERR_ARG, "L and NK must be >= 1");
+  if (!cmap || !bt) return set_error(HIPNUFFT_ERR_ARG, "null interpolator pointer");
+  if (p->M % NK != 0)
+    return set_error(HIPNUFFT_ERR_ARG, "the samples per shot NK must divide the number of samples");
+  if ((int64_t)B * C * L > (int64_t)1 << 30) return set_error(HIPNUFFT_ERR_ARG, "too many transforms");
+  return make_chunks(p, sense ? B : B * C, (sense ? C : 1) * L, true, chunks);
+}
+
+}  // namespace
+
+extern "C" {
+
+int hipnufft_orc_op(hipnufft_plan* p, const void* img, const void* smaps, const void* cmap,
+                    const void* bt, int L, int64_t NK, void* ksp, int B, int C, float scale) {
+  if (!p) return set_error(HIPNUFFT_ERR_ARG, "null plan");
+  if (!img || !ksp) return set_error(HIPNUFFT_ERR_ARG, "null data pointer");
+  std::vector<Chunk> chunks;
+  int rc = orc_setup(p, cmap, bt, L, NK, B, C, smaps != nullptr, chunks);
+  if (rc) return rc;
+  if ((rc = ensure_kwork(p))) return rc;
+  const float2* x = (const float2*)img;
+  const float2* s = (const float2*)smaps;
+  const float2* cm = (const float2*)cmap;
+  const float2* b = (const float2*)bt;
+  for (const Chunk& c : chunks) {
+    if ((rc = fwd_to_grid(p, x, s, C, c.t0, c.T, 1.0f, p->sign2, 0, 0, cm, L))) return rc;
+    if ((rc = launch_interp(p, p->grid, grid_size(p), SetRange{0, 1, c.T}, p->kwork, nullptr,
+                            nullptr, scale, 0)))
+      return rc;
+    if ((rc = orc_combine(p, p->kwork, b, L, NK, c.t0, c.T, (float2*)ksp))) return rc;
+  }
+  return 0;
+}
+
+int hipnufft_orc_adj(hipnufft_plan* p, const void* ksp, const void* smaps, const void* cmap,
+                     const void* bt, int L, int64_t NK, const float* density, void* img, int B,
+                     int C, float scale) {
+  if (!p) return set_error(HIPNUFFT_ERR_ARG, "null plan");
+  if (!img || !ksp) return set_error(HIPNUFFT_ERR_ARG, "null data pointer");
+  std::vector<Chunk> chunks;
+  int rc = orc_setup(p, cmap, bt, L, NK, B, C, smaps != nullptr, chunks);
+  if (rc) return rc;
+  if ((rc = ensure_ksorted(p))) return rc;
+  const float2* s = (const float2*)smaps;
+  const float2* cm = (const float2*)cmap;
+  const float2* b = (const float2*)bt;
+  for (const Chunk& c : chunks) {
+    if ((rc = orc_gather(p, (const float2*)ksp, b, L, NK, density, c.t0, c.T, p->ksorted)))
+      return rc;
+    if ((rc = launch_spread_sorted(p, p->ksorted, SetRange{0, 1, c.T}, p->grid, grid_size(p))))
+      return rc;
+    if ((rc = grid_to_img(p, (float2*)img, s, C, c.t0, c.T, scale, -p->sign2, 0, cm, L)))
+      return rc;
+  }
+  return 0;
+}
+
 int hipnufft_execute(hipnufft_plan* p, const void* in, void* out) {
   if (!p) return set_error(HIPNUFFT_ERR_ARG, "null plan");
   const int B = p->nsets > 1 ? p->nsets : p->ntrans;

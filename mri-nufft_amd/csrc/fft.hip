@@ -211,6 +211,10 @@ struct RowsArgs {
   int C, t0, T;
   int conj_smaps;  // forward: multiply by conj(smap) instead of smap
   int smaps_all;   // smaps hold one map per transform ([B][C]) instead of [C]
+  // optional second factor cmap[L][N]: transforms are ordered [B][C][L] and
+  // transform tt also multiplies by cmap[tt % L] (off-resonance interpolators)
+  const float2* cmap;
+  int L;
   float2* grid;
   int64_t gstride;
   int M0, M1, M2;
@@ -243,9 +247,12 @@ __global__ __launch_bounds__(kThreads) void k_rows_fwd(RowsArgs a) {
   const int r = r0 + l;
   const bool rok = r < rows;
   const int i0 = rok ? r / a.M1 : 0, i1 = rok ? r - i0 * a.M1 : 0;
-  const float2* src = a.img + (int64_t)(a.smaps ? tt / a.C : tt) * N + (int64_t)r * a.M2;
+  const int per_img = (a.smaps ? a.C : 1) * a.L;  // transforms of one image
+  const float2* src = a.img + (int64_t)(tt / per_img) * N + (int64_t)r * a.M2;
   const float2* sm =
-      a.smaps ? a.smaps + (int64_t)(a.smaps_all ? tt : tt % a.C) * N + (int64_t)r * a.M2 : nullptr;
+      a.smaps ? a.smaps + (int64_t)(a.smaps_all ? tt : (tt / a.L) % a.C) * N + (int64_t)r * a.M2
+              : nullptr;
+  const float2* cm = a.cmap ? a.cmap + (int64_t)(tt % a.L) * N + (int64_t)r * a.M2 : nullptr;
   const float rs = rok ? a.dec0[i0] * a.dec1[i1] * a.scale : 0.f;
   const int M2 = a.M2, h2 = M2 / 2;
 
@@ -259,6 +266,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_fwd(RowsArgs a) {
     if (rok && x < M2) {
       v[j] = src[x];
       if (sm) sv[j] = sm[x];
+      if (cm) v[j] = cmul(v[j], cm[x]);
     }
   }
 #pragma unroll
@@ -301,18 +309,13 @@ __global__ __launch_bounds__(kThreads) void k_rows_adj(RowsArgs a) {
   const int r0 = bi.x * L;
   const int n = a.nf2;
   const int64_t N = (int64_t)rows * a.M2;
-  int tb, te, out_img;
-  if (a.smaps) {  // one output image per batch present in the chunk
-    const int b = a.t0 / a.C + bi.y;
-    out_img = b;
-    tb = max(a.t0, b * a.C);
-    te = min(a.t0 + a.T, (b + 1) * a.C);
-  } else {
-    out_img = a.t0 + bi.y;
-    tb = out_img;
-    te = out_img + 1;
-  }
-  const bool add_existing = a.smaps && (tb % a.C != 0);
+  // one output image per image present in the chunk, the sum of its
+  // per_img consecutive transforms (coils and / or interpolators)
+  const int per_img = (a.smaps ? a.C : 1) * a.L;
+  const int out_img = a.t0 / per_img + bi.y;
+  const int tb = max(a.t0, out_img * per_img);
+  const int te = min(a.t0 + a.T, (out_img + 1) * per_img);
+  const bool add_existing = tb % per_img != 0;
   // line-major thread mapping: thread (l, x0) handles positions x0 + j * TPL
   // of line l, so it keeps one row base across the coil loop
   const int TPL = kThreads >> a.lgL;  // threads per line (n <= TPL * EPT)
@@ -351,7 +354,9 @@ __global__ __launch_bounds__(kThreads) void k_rows_adj(RowsArgs a) {
     asm volatile("" : "+s"(lg), "+s"(lp));
     fft_lines<S, EPT>(buf, twl, a.ax, lg, lp);
     const float2* sm =
-        a.smaps ? a.smaps + (int64_t)(a.smaps_all ? tt : tt % a.C) * N + (int64_t)r * a.M2 : nullptr;
+        a.smaps ? a.smaps + (int64_t)(a.smaps_all ? tt : (tt / a.L) % a.C) * N + (int64_t)r * a.M2
+                : nullptr;
+    const float2* cm = a.cmap ? a.cmap + (int64_t)(tt % a.L) * N + (int64_t)r * a.M2 : nullptr;
 #pragma unroll
     for (int j = 0; j < EPT; ++j) {
       const int x = x0 + j * TPL;
@@ -363,6 +368,11 @@ __global__ __launch_bounds__(kThreads) void k_rows_adj(RowsArgs a) {
             float2 sv = sm[x];
             sv.y = -sv.y;
             v = cmul(v, sv);
+          }
+          if (cm) {
+            float2 cv = cm[x];
+            cv.y = -cv.y;
+            v = cmul(v, cv);
           }
         }
         accl[x] = (tt == tb) ? v : cadd(accl[x], v);
@@ -750,6 +760,7 @@ RowsArgs rows_args(const hipnufft_plan_s* p) {
   a.dec2 = p->dec[2];
   a.ax = p->fft[2];
   a.lgL = lg2(p->fft[2].L);
+  a.L = 1;
   return a;
 }
 
@@ -781,10 +792,13 @@ int mul_spectrum(const hipnufft_plan_s* p, const float2* K, int T) {
 namespace {
 
 int rows_fwd(const hipnufft_plan_s* p, const float2* img, const float2* smaps, int C, int t0, int T,
-             float scale, int sign, int conj_smaps, int smaps_all) {
+             float scale, int sign, int conj_smaps, int smaps_all, const float2* cmap = nullptr,
+             int nmaps = 1) {
   RowsArgs a = rows_args(p);
   a.conj_smaps = conj_smaps;
   a.smaps_all = smaps_all;
+  a.cmap = cmap;
+  a.L = nmaps;
   a.img = img;
   a.smaps = smaps;
   a.C = C;
@@ -809,18 +823,21 @@ int rows_fwd(const hipnufft_plan_s* p, const float2* img, const float2* smaps, i
 }
 
 int rows_adj(const hipnufft_plan_s* p, float2* img, const float2* smaps, int C, int t0, int T,
-             float scale, int sign, int smaps_all) {
+             float scale, int sign, int smaps_all, const float2* cmap = nullptr, int nmaps = 1) {
   RowsArgs a = rows_args(p);
   a.img_out = img;
   a.smaps = smaps;
   a.smaps_all = smaps_all;
+  a.cmap = cmap;
+  a.L = nmaps;
   a.C = C;
   a.t0 = t0;
   a.T = T;
   a.scale = scale;
   const int rows = a.M0 * a.M1;
   const int L = p->fft[2].L;
-  const int nimg = smaps ? ((t0 + T - 1) / C - t0 / C + 1) : T;
+  const int per_img = (smaps ? C : 1) * nmaps;
+  const int nimg = (t0 + T - 1) / per_img - t0 / per_img + 1;
   const dim3 grd((unsigned)((rows + L - 1) / L), (unsigned)nimg);
   const size_t lds = rows_adj_lds(p->fft[2], a.M2);
   StageScope timer(p, kStageRowsAdj);
@@ -837,9 +854,10 @@ int rows_adj(const hipnufft_plan_s* p, float2* img, const float2* smaps, int C, 
 }  // namespace
 
 int fwd_to_grid(const hipnufft_plan_s* p, const float2* img, const float2* smaps, int C, int t0,
-                int T, float scale, int sign, int conj_smaps, int smaps_all) {
+                int T, float scale, int sign, int conj_smaps, int smaps_all, const float2* cmap,
+                int L) {
   if (T <= 0) return 0;
-  int rc = rows_fwd(p, img, smaps, C, t0, T, scale, sign, conj_smaps, smaps_all);
+  int rc = rows_fwd(p, img, smaps, C, t0, T, scale, sign, conj_smaps, smaps_all, cmap, L);
   if (rc) return rc;
   if (p->ndim >= 2 &&
       (rc = strided(p, 1, T, (int)p->N[1], 0, (int)p->N[0], sign, kStageColsFwd)))
@@ -850,7 +868,7 @@ int fwd_to_grid(const hipnufft_plan_s* p, const float2* img, const float2* smaps
 }
 
 int grid_to_img(const hipnufft_plan_s* p, float2* img, const float2* smaps, int C, int t0, int T,
-                float scale, int sign, int smaps_all) {
+                float scale, int sign, int smaps_all, const float2* cmap, int L) {
   if (T <= 0) return 0;
   int rc;
   if (p->ndim == 3 && (rc = strided(p, 0, T, 0, (int)p->N[0], 0, sign, kStageSlabsAdj)))
@@ -858,7 +876,7 @@ int grid_to_img(const hipnufft_plan_s* p, float2* img, const float2* smaps, int 
   if (p->ndim >= 2 &&
       (rc = strided(p, 1, T, 0, (int)p->N[1], (int)p->N[0], sign, kStageColsAdj)))
     return rc;
-  return rows_adj(p, img, smaps, C, t0, T, scale, sign, smaps_all);
+  return rows_adj(p, img, smaps, C, t0, T, scale, sign, smaps_all, cmap, L);
 }
 
 int gram_chain(const hipnufft_plan_s* p, const float2* img, const float2* smaps, float2* out,

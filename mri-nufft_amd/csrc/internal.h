@@ -263,16 +263,32 @@ int launch_grad_traj(const hipnufft_plan_s* p, const float2* grid,
 
 // ---- fft.hip -------------------------------------------------------------
 // Forward chain image(s) -> fine grid (deconvolve, pad, all FFT passes).
-// Transform tt = t0 + t of the chunk uses image (smaps ? tt/C : tt) and map
-// smaps[smaps_all ? tt : tt % C].
+// Transforms are ordered [image][coil][l] with L interpolator maps cmap[L][N]
+// (L = 1 and no cmap for plain SENSE).  Transform tt = t0 + t of the chunk
+// uses image tt / per_img (per_img = (smaps ? C : 1) * L), map
+// smaps[smaps_all ? tt : (tt / L) % C] and cmap[tt % L].
 int fwd_to_grid(const hipnufft_plan_s* p, const float2* img,
                 const float2* smaps, int C, int t0, int T, float scale,
-                int sign, int conj_smaps = 0, int smaps_all = 0);
-// Adjoint chain fine grid -> image(s) (all FFT passes, crop, deconvolve and,
-// with smaps, conjugate coil combine).  With smaps the chunk [t0, t0+T)
-// covers batches t0/C ... (partial batches are accumulated into img).
+                int sign, int conj_smaps = 0, int smaps_all = 0,
+                const float2* cmap = nullptr, int L = 1);
+// Adjoint chain fine grid -> image(s) (all FFT passes, crop, deconvolve and
+// the conjugate combine over the per_img transforms of each image).  Images
+// only partly in the chunk [t0, t0+T) are accumulated into img.
 int grid_to_img(const hipnufft_plan_s* p, float2* img, const float2* smaps,
-                int C, int t0, int T, float scale, int sign, int smaps_all = 0);
+                int C, int t0, int T, float scale, int sign, int smaps_all = 0,
+                const float2* cmap = nullptr, int L = 1);
+
+// ---- orc.hip -------------------------------------------------------------
+// Off-resonance k-space passes for transforms [t0, t0+T) ordered
+// [image][coil][l] (L interpolators, NK samples per shot, bt = B[NK][L]).
+// combine: y[q] (+)= sum_l B[:, l] * kv[q L + l - t0] for each output row q
+// present (kv: the chunk's interpolated k-space, original order); rows
+// whose l range starts at 0 are overwritten, the others accumulated.
+int orc_combine(const hipnufft_plan_s* p, const float2* kv, const float2* bt, int L, int64_t NK,
+                int t0, int T, float2* y);
+// gather: out[t] (sorted order) = y[tt / L] * conj(B[:, tt % L]) * density
+int orc_gather(const hipnufft_plan_s* p, const float2* y, const float2* bt, int L, int64_t NK,
+               const float* density, int t0, int T, float2* out);
 
 // rows independent FP64 dots of a[rows][n] and b[rows][n] -> out[rows][2]
 // grid[t] *= K (elementwise complex) for the T grids of the plan workspace

@@ -11,6 +11,9 @@ every transform raises.
 
 from .base import FourierOperatorBase, check_backend, get_operator, list_backends, power_method
 from .density import get_density, pipe
+from .field_map import (get_complex_fieldmap_rad, get_orc_factorization, make_b0map,
+                        make_t2smap)
+from .off_resonance import MRIFourierCorrected
 from .operator import MRIHipNUFFT
 from .optim import cg, get_optimizer, lsmr, lsqr
 from .smaps import get_smaps, low_frequency
@@ -18,19 +21,24 @@ from .stacked import MRIStackedHipNUFFT, MRIStackedNUFFT, stacked2traj3d, traj3d
 
 __all__ = [
     "FourierOperatorBase",
+    "MRIFourierCorrected",
     "MRIHipNUFFT",
     "MRIStackedHipNUFFT",
     "MRIStackedNUFFT",
     "cg",
     "check_backend",
+    "get_complex_fieldmap_rad",
     "get_density",
     "get_operator",
+    "get_orc_factorization",
     "get_optimizer",
     "get_smaps",
     "low_frequency",
     "list_backends",
     "lsmr",
     "lsqr",
+    "make_b0map",
+    "make_t2smap",
     "pipe",
     "power_method",
     "register_into_mrinufft",

@@ -20,7 +20,7 @@ CSRC = ROOT / "csrc"
 INCLUDE = REPO / "include"
 OBJ = ROOT / "_build"
 LIB = PKG / "_lib" / "libhipnufft.so"
-SOURCES = ["sort.hip", "spread_interp.hip", "fft.hip", "maps.hip", "capi.hip"]
+SOURCES = ["sort.hip", "spread_interp.hip", "fft.hip", "maps.hip", "orc.hip", "capi.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("HIPNUFFT_ARCH", "gfx950")
 

@@ -69,6 +69,8 @@ _SIGNATURES = {
     "hipnufft_sense_op": [_P, _P, _P, _P, _I, _I, _F],
     "hipnufft_sense_adj": [_P, _P, _P, _P, _P, _I, _I, _F],
     "hipnufft_sense_dc": [_P, _P, _P, _P, _P, _P, _I, _I, _F],
+    "hipnufft_orc_op": [_P, _P, _P, _P, _P, _I, _I64, _P, _I, _I, _F],
+    "hipnufft_orc_adj": [_P, _P, _P, _P, _P, _I, _I64, _P, _P, _I, _I, _F],
     "hipnufft_sense_grad_traj": [_P, _P, _P, _P, _I, _I, _F, _I, _I, _P],
     "hipnufft_dot_c64": [_P, _P, _I64, _I, _P, _P, _P],
     "hipnufft_dot_c64_rows": [_P, _P, _I64, _I64, _I, _P, _P, _P],
@@ -294,6 +296,23 @@ class RawPlan:
         self._sync_stream()
         check(lib().hipnufft_sense_dc(self._h, ptr(img), ptr(smaps), ptr(obs), ptr(density),
                                       ptr(grad), B, C, scale), "data_consistency")
+
+    def orc_op(self, img, smaps, cmap, bt, ksp, B, C, scale):
+        """Off-resonance forward: cmap (L, *shape), bt (NK, L) complex64 on the device."""
+        self._sync_stream()
+        L, NK = int(cmap.shape[0]), int(bt.shape[0])
+        if bt.shape[1] != L:
+            raise ValueError("the temporal interpolators must be (NK, L)")
+        check(lib().hipnufft_orc_op(self._h, ptr(img), ptr(smaps), ptr(cmap), ptr(bt), L, NK,
+                                    ptr(ksp), B, C, scale), "orc op")
+
+    def orc_adj(self, ksp, smaps, cmap, bt, density, img, B, C, scale):
+        self._sync_stream()
+        L, NK = int(cmap.shape[0]), int(bt.shape[0])
+        if bt.shape[1] != L:
+            raise ValueError("the temporal interpolators must be (NK, L)")
+        check(lib().hipnufft_orc_adj(self._h, ptr(ksp), ptr(smaps), ptr(cmap), ptr(bt), L, NK,
+                                     ptr(density), ptr(img), B, C, scale), "orc adj_op")
 
     def sense_grad_traj(self, img, smaps, dy, B, C, scale, isign, conj_smaps, gtraj):
         self._sync_stream()

@@ -308,6 +308,13 @@ class FourierOperatorBase:
             self.squeeze_dims = sq
         return lipschitz_cst
 
+    def with_off_resonance_correction(self, readout_time, b0_map=None, r2star_map=None, mask=None,
+                                      interpolator="svd"):
+        """A new operator with off-resonance correction (``base.py:385-398``)."""
+        from .off_resonance import MRIFourierCorrected
+
+        return MRIFourierCorrected(self, b0_map, readout_time, r2star_map, mask, interpolator)
+
     def pinv_solver(self, kspace_data, optim="lsqr", **kwargs):
         """Least-squares reconstruction with a registered optimizer."""
         from .optim import get_optimizer
