@@ -182,7 +182,8 @@ class MRIFourierCorrected(FourierOperatorBase):
         need = fo.n_batchs * fo.n_coils * self.n_interpolators
         nf = [2 * s for s in fo.shape]
         grid = 8 * int(np.prod(nf)) * max(1, int(fo._plan_opts["upsampfac"] / 2.0) ** 2)
-        T = int(max(1, min(need, DEFAULT_WORKSPACE_BYTES // max(grid, 1))))
+        tmax = int(max(1, min(need, DEFAULT_WORKSPACE_BYTES // max(grid, 1))))
+        T = -(-need // -(-need // tmax))  # equal chunks: no short tail launch
         if fo.raw_op.n_trans >= T:
             return fo.raw_op.plan
         key = (T, id(fo._samples))
