@@ -55,7 +55,8 @@ typedef struct hipnufft_opts {
   int max_subproblem;   /* max points per interp work item (dense bins split) */
   int deterministic;    /* reserved (spreading is always fixed-order)        */
   int spread_max;       /* max list entries per spread work item (dense
-                           tiles are split and their partials summed)        */
+                           tiles are split and their partials summed;
+                           0: 1024 in 3D, else 4096)                         */
   int spread_method;    /* HIPNUFFT_SPREAD_*: which spreading kernel         */
   int interp_span;      /* bins along the last axis staged together by one
                            interpolation workgroup (0: 2 in 3D, else 1)      */
@@ -75,7 +76,9 @@ enum {
   HIPNUFFT_SPREAD_COLUMNS = 3, /* lane per 16-cell column, 16 transforms */
   HIPNUFFT_SPREAD_MATRIX = 4   /* 4^3 blocks on the matrix cores (FP32
                                   MFMA); same order every run, equal to the
-                                  others to rounding, not bit for bit */
+                                  others to rounding, not bit for bit; 8
+                                  transforms per workgroup, or 16 (same
+                                  bits) with spread_group >= 16 */
 };
 
 /* Fill *opts with defaults.  Replaces the cufinufft Plan **kwargs

@@ -144,6 +144,9 @@ int make_axis(int64_t n, FFTAxis* ax, size_t* bytes, bool allow_direct = false) 
   ax->direct = 0;
   while (m % 8 == 0 && ax->nrad < kMaxRadices) { ax->rad[ax->nrad++] = 8; m /= 8; }
   if (m % 4 == 0) { ax->rad[ax->nrad++] = 4; m /= 4; }
+  // a remaining factor 2 pairs with a 3 into one radix-6 pass (one LDS round
+  // trip fewer): 384 = 8 * 8 * 6
+  if (m % 6 == 0) { ax->rad[ax->nrad++] = 6; m /= 6; }
   if (m % 2 == 0) { ax->rad[ax->nrad++] = 2; m /= 2; }
   while (m % 3 == 0 && ax->nrad < kMaxRadices) { ax->rad[ax->nrad++] = 3; m /= 3; }
   while (m % 5 == 0 && ax->nrad < kMaxRadices) { ax->rad[ax->nrad++] = 5; m /= 5; }
@@ -523,7 +526,9 @@ int hipnufft_plan_create(int type, int ndim, const int64_t* nmodes, int iflag, i
   p->spread_group = o.spread_group > 0 ? o.spread_group : 0;  // 0: chosen per launch
   p->interp_group = o.interp_group > 0 ? o.interp_group : (ndim == 3 ? 1 : 4);
   p->max_sub = o.max_subproblem > 0 ? o.max_subproblem : 1024;
-  p->spread_max = o.spread_max > 0 ? o.spread_max : 4096;
+  // 3D: dense tiles (a radial k-space centre holds ~80 K list entries per
+  // tile) split finely, so that their work items do not form a tail
+  p->spread_max = o.spread_max > 0 ? o.spread_max : (ndim == 3 ? 1024 : 4096);
   if (o.spread_method < HIPNUFFT_SPREAD_AUTO || o.spread_method > HIPNUFFT_SPREAD_MATRIX) {
     free_plan(p);
     return set_error(HIPNUFFT_ERR_ARG, "unknown spread_method");

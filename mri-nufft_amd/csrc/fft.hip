@@ -107,12 +107,30 @@ __device__ __forceinline__ void dft5(float2* v) {
   v[3] = csub(p2, q2);
 }
 
+// 6 = 2 x 3 by the prime-factor mapping (no inner twiddles): inputs
+// (3 n1 + 2 n2) mod 6 go through two 3-point DFTs over n2, then 2-point
+// DFTs over n1, and (k1, k2) lands at (3 k1 + 4 k2) mod 6
+template <int S>
+__device__ __forceinline__ void dft6(float2* v) {
+  float2 a[3] = {v[0], v[2], v[4]};
+  float2 b[3] = {v[3], v[5], v[1]};
+  dft3<S>(a);
+  dft3<S>(b);
+  v[0] = cadd(a[0], b[0]);
+  v[3] = csub(a[0], b[0]);
+  v[4] = cadd(a[1], b[1]);
+  v[1] = csub(a[1], b[1]);
+  v[2] = cadd(a[2], b[2]);
+  v[5] = csub(a[2], b[2]);
+}
+
 template <int R, int S>
 __device__ __forceinline__ void dftR(float2* v) {
   if constexpr (R == 2) dft2<S>(v);
   if constexpr (R == 3) dft3<S>(v);
   if constexpr (R == 4) dft4<S>(v);
   if constexpr (R == 5) dft5<S>(v);
+  if constexpr (R == 6) dft6<S>(v);
   if constexpr (R == 8) dft8<S>(v);
 }
 
@@ -183,6 +201,7 @@ __device__ void fft_lines(float2* buf, const float2* twl, const FFTAxis& ax, int
       case 3: stockham_pass<3, S, EPT>(buf, ax.n, lgL, LP, Ns, mg, twl); break;
       case 4: stockham_pass<4, S, EPT>(buf, ax.n, lgL, LP, Ns, mg, twl); break;
       case 5: stockham_pass<5, S, EPT>(buf, ax.n, lgL, LP, Ns, mg, twl); break;
+      case 6: stockham_pass<6, S, EPT>(buf, ax.n, lgL, LP, Ns, mg, twl); break;
       default: stockham_pass<8, S, EPT>(buf, ax.n, lgL, LP, Ns, mg, twl); break;
     }
     Ns *= R;

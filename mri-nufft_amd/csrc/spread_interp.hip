@@ -484,21 +484,28 @@ __global__ __launch_bounds__(kThreads) void k_spread_blocks(SpreadArgs a) {
   }
 }
 
-// Matrix-core spreading for the tiles of the block kernel, 8 transforms per
-// workgroup.  For one 4 x 4 x 4 block and four entries touching it, the sums
-// of the block's cells are a 64 x 4 by 4 x 16 product: cells by entries
-// (the weight products w0 w1 w2) times entries by values (the strengths, 8
-// transforms x re / im).  Each slice of 16 cells (fixed first offset) is one
-// v_mfma_f32_16x16x4f32: lane l supplies A[l % 16][l / 16] (cell l % 16 of
-// the slice, entry l / 16), B[l / 16][l % 16] (entry l / 16, value l % 16)
-// and holds D[4 (l / 16) + r][l % 16] in register r.  A wave takes the
-// entries touching each of its blocks four at a time in list order (a
-// ballot per 64 staged entries), so every cell's sum has the same order on
-// every run: deterministic, equal to the other kernels to rounding.
+// Matrix-core spreading for the tiles of the block kernel, 8 * GH transforms
+// per workgroup of 4 * GH waves.  For one 4 x 4 x 4 block and four entries
+// touching it, the sums of the block's cells are a 64 x 4 by 4 x 16 product
+// per 8 transforms: cells by entries (the weight products w0 w1 w2) times
+// entries by values (the strengths, 8 transforms x re / im).  Each slice of
+// 16 cells (fixed first offset) is one v_mfma_f32_16x16x4f32 per 8
+// transforms: lane l supplies A[l % 16][l / 16] (cell l % 16 of the slice,
+// entry l / 16), B[l / 16][l % 16] (entry l / 16, value l % 16) and holds
+// D[4 (l / 16) + r][l % 16] in register r.  With GH = 2 the A operand (and
+// the staging and the entry selection behind it) serves both halves of 16
+// transforms.  A wave takes the entries touching each of its blocks four at
+// a time in list order (a ballot per 64 staged entries, windows aligned the
+// same way for every GH), so every cell's sum has the same order on every
+// run and for both GH: deterministic, equal to the other kernels to
+// rounding.
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-__global__ __launch_bounds__(kThreads, 4) void k_spread_mfma(SpreadArgs a) {
-  constexpr int G = 8;
+template <int GH>
+__global__ __launch_bounds__(4 * 64 * GH, 4 / GH) void k_spread_mfma(SpreadArgs a) {
+  constexpr int G = 8 * GH;              // transforms per workgroup
+  constexpr int NWAVE = 4 * GH;          // waves per workgroup
+  constexpr int BPW = kBlkPerWave / GH;  // blocks per wave (16 blocks at most)
   extern __shared__ float smem[];
   const int W = a.g.w;
   const int WP = W + 2 * kBlkPad;
@@ -527,38 +534,41 @@ __global__ __launch_bounds__(kThreads, 4) void k_spread_mfma(SpreadArgs a) {
   const int lane = (int)threadIdx.x & 63;
   const int kq = lane >> 4;               // entry of the group (A column, B row)
   const int cy = (lane >> 2) & 3, cz = lane & 3;  // A row: cell (slice, cy, cz)
-  const int nbw = nb[0] * nb[1] * nb[2] / 4;
-  int bidx[kBlkPerWave][3];
-  uint32_t bbit[kBlkPerWave];
+  const int nblk = nb[0] * nb[1] * nb[2];
+  int bidx[BPW][3];
+  uint32_t bbit[BPW];
 #pragma unroll
-  for (int bi = 0; bi < kBlkPerWave; ++bi) {
-    const int b = wave + 4 * bi;
+  for (int bi = 0; bi < BPW; ++bi) {
+    const int b = wave + NWAVE * bi;
     bidx[bi][2] = b % nb[2];
     bidx[bi][1] = (b / nb[2]) % nb[1];
     bidx[bi][0] = b / (nb[2] * nb[1]);
-    bool live = bi < nbw;
+    bool live = b < nblk;
 #pragma unroll
     for (int d = 0; d < 3; ++d) live = live && bidx[bi][d] * kBlkE < ext[d];
     bbit[bi] = live ? 1u << b : 0u;
   }
 
-  f32x4 acc[kBlkPerWave][kBlkE];
+  f32x4 acc[BPW][kBlkE][GH];
 #pragma unroll
-  for (int bi = 0; bi < kBlkPerWave; ++bi)
+  for (int bi = 0; bi < BPW; ++bi)
 #pragma unroll
-    for (int s = 0; s < kBlkE; ++s) acc[bi][s] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int s = 0; s < kBlkE; ++s)
+#pragma unroll
+      for (int h = 0; h < GH; ++h) acc[bi][s][h] = f32x4{0.f, 0.f, 0.f, 0.f};
 
+  constexpr int CHUNK = kBlkChunk * GH;  // staged entries per round (a multiple of 64)
   const int e0 = a.it_begin[item];
   const int e1 = min(e0 + a.smax, a.list_start[key + 1]);
-  for (int eb = e0; eb < e1; eb += kBlkChunk) {
-    const int n = min(kBlkChunk, e1 - eb);
+  for (int eb = e0; eb < e1; eb += CHUNK) {
+    const int n = min(CHUNK, e1 - eb);
     stage_blk_entries<G>(a, smem, eb, n, PW, WP, org, ext, nf, nb, t0, Gv, inv_hw, beta);
     __syncthreads();
     for (int base = 0; base < n; base += 64) {
       const int qi = base + lane;
       const uint32_t hm = qi < n ? reinterpret_cast<const uint32_t*>(smem + qi * PW)[0] : 0u;
 #pragma unroll
-      for (int bi = 0; bi < kBlkPerWave; ++bi) {
+      for (int bi = 0; bi < BPW; ++bi) {
         if (!bbit[bi]) continue;  // uniform
         uint64_t todo = __ballot((hm & bbit[bi]) != 0u);
         while (todo) {  // uniform: four entries per step, list order
@@ -570,7 +580,9 @@ __global__ __launch_bounds__(kThreads, 4) void k_spread_mfma(SpreadArgs a) {
           }
           const int e = kq == 0 ? ek[0] : kq == 1 ? ek[1] : kq == 2 ? ek[2] : ek[3];
           float av[kBlkE] = {0.f, 0.f, 0.f, 0.f};
-          float bv = 0.f;
+          float bv[GH];
+#pragma unroll
+          for (int h = 0; h < GH; ++h) bv[h] = 0.f;
           if (e >= 0) {
             const float* sp = smem + (base + e) * PW;
             const uint32_t* hd = reinterpret_cast<const uint32_t*>(sp);
@@ -580,50 +592,58 @@ __global__ __launch_bounds__(kThreads, 4) void k_spread_mfma(SpreadArgs a) {
             const float wyz = sp[oy + cy] * sp[oz + cz];
 #pragma unroll
             for (int sx = 0; sx < kBlkE; ++sx) av[sx] = sp[ox + sx] * wyz;
-            bv = sp[kBlkHead + (lane & 15)];
+#pragma unroll
+            for (int h = 0; h < GH; ++h) bv[h] = sp[kBlkHead + 16 * h + (lane & 15)];
           }
 #pragma unroll
           for (int sx = 0; sx < kBlkE; ++sx)
-            acc[bi][sx] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[sx], bv, acc[bi][sx], 0, 0, 0);
+#pragma unroll
+            for (int h = 0; h < GH; ++h)
+              acc[bi][sx][h] =
+                  __builtin_amdgcn_mfma_f32_16x16x4f32(av[sx], bv[h], acc[bi][sx][h], 0, 0, 0);
         }
       }
     }
     __syncthreads();
   }
 
-  // register r of lane l holds value l % 16 (transform (l % 16) / 2, re / im
-  // by parity) of cell (slice, l / 16, r); lanes 2g and 2g + 1 swap halves
-  // so that each writes two whole complex cells
+  // register r of lane l holds value l % 16 (transform 8 h + (l % 16) / 2,
+  // re / im by parity) of cell (slice, l / 16, r); lanes 2g and 2g + 1 swap
+  // halves so that each writes two whole complex cells
   const int TC = td[0] * td[1] * td[2];
-  const int g = (lane & 15) >> 1;
   const bool odd = lane & 1;
   const int ry = lane >> 4;
 #pragma unroll
-  for (int bi = 0; bi < kBlkPerWave; ++bi) {
+  for (int bi = 0; bi < BPW; ++bi) {
     if (!bbit[bi]) continue;
 #pragma unroll
     for (int sx = 0; sx < kBlkE; ++sx) {
-      float2 out[2];
 #pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        // even lane keeps cells r = 0, 1, odd lane cells r = 2, 3
-        const float mine = odd ? acc[bi][sx][2 + h] : acc[bi][sx][h];
-        const float give = odd ? acc[bi][sx][h] : acc[bi][sx][2 + h];
-        const float other = __shfl_xor(give, 1);
-        out[h] = odd ? make_float2(other, mine) : make_float2(mine, other);
-      }
-      if (g >= Gv) continue;
-      const int c0 = bidx[bi][0] * kBlkE + sx, c1 = bidx[bi][1] * kBlkE + ry;
+      for (int hh = 0; hh < GH; ++hh) {
+        float2 out[2];
 #pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const int c2 = bidx[bi][2] * kBlkE + (odd ? 2 : 0) + h;
-        if (c0 >= ext[0] || c1 >= ext[1] || c2 >= ext[2]) continue;
-        if (slot >= 0) {
-          const int r = (c0 * td[1] + c1) * td[2] + c2;
-          a.part[((int64_t)slot * a.ntrans + tb + t0 + g) * TC + r] = out[h];
-        } else {
-          const int64_t gi = ((int64_t)(org[0] + c0) * nf[1] + org[1] + c1) * nf[2] + org[2] + c2;
-          a.grid[(int64_t)(tb + t0 + g) * a.grid_stride + gi] = out[h];
+        for (int h = 0; h < 2; ++h) {
+          // even lane keeps cells r = 0, 1, odd lane cells r = 2, 3
+          const float mine = odd ? acc[bi][sx][hh][2 + h] : acc[bi][sx][hh][h];
+          const float give = odd ? acc[bi][sx][hh][h] : acc[bi][sx][hh][2 + h];
+          const float other = __shfl_xor(give, 1);
+          out[h] = odd ? make_float2(other, mine) : make_float2(mine, other);
+        }
+        const int g = 8 * hh + ((lane & 15) >> 1);
+        if (g >= Gv) continue;
+        const int c0 = bidx[bi][0] * kBlkE + sx, c1 = bidx[bi][1] * kBlkE + ry;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int c2 = bidx[bi][2] * kBlkE + (odd ? 2 : 0) + h;
+          if (c0 >= ext[0] || c1 >= ext[1] || c2 >= ext[2]) continue;
+          if (slot >= 0) {
+            const int r = (c0 * td[1] + c1) * td[2] + c2;
+            a.part[((int64_t)slot * a.ntrans + tb + t0 + g) * TC + r] = out[h];
+          } else {
+            const int64_t gi =
+                ((int64_t)(org[0] + c0) * nf[1] + org[1] + c1) * nf[2] + org[2] + c2;
+            a.grid[(int64_t)(tb + t0 + g) * a.grid_stride + gi] = out[h];
+          }
         }
       }
     }
@@ -1338,12 +1358,21 @@ int launch_spread_sorted(const hipnufft_plan_s* p, const float2* in, SetRange r,
                      "block spreading needs 3D tiles of 4, 8 or 16 blocks of 4^3 cells, "
                      "at most 4 along an axis");
   if (method == HIPNUFFT_SPREAD_MATRIX) {
-    a.G = 8;
-    const size_t lds = sizeof(float) * (size_t)kBlkChunk * blk_record_floats(p->w, 8);
+    // 16 transforms per workgroup (8 waves) when asked for (spread_group >=
+    // 16) and a launch has more than 8; measured no faster than 8 at the 3D
+    // headline, so 8 by default
+    const int GH = (r.ts > 8 && p->spread_group >= 16) ? 2 : 1;
+    a.G = 8 * GH;
+    const size_t lds = sizeof(float) * (size_t)kBlkChunk * GH * blk_record_floats(p->w, 8 * GH);
     if (nitems > 0) {
-      const dim3 grd((unsigned)nitems, (unsigned)((r.ts + 7) / 8));
+      const dim3 grd((unsigned)nitems, (unsigned)((r.ts + 8 * GH - 1) / (8 * GH)));
       StageScope timer(p, kStageSpread);
-      hipLaunchKernelGGL(k_spread_mfma, grd, dim3(kThreads), lds, p->stream, a);
+      if (GH == 2) {
+        if ((rc = allow_lds(k_spread_mfma<2>, lds))) return rc;
+        hipLaunchKernelGGL(k_spread_mfma<2>, grd, dim3(2 * kThreads), lds, p->stream, a);
+      } else {
+        hipLaunchKernelGGL(k_spread_mfma<1>, grd, dim3(kThreads), lds, p->stream, a);
+      }
       HN_LAUNCH_CHECK();
     }
     if (ndense > 0) {
