@@ -303,6 +303,131 @@ __host__ __device__ constexpr int blk_record_floats(int W, int G) {
   return (kBlkHead + 2 * G + 3 * (W + 2 * kBlkPad) + 3) & ~3;
 }
 
+// Where one list entry of a tile sits, as loaded from HBM for the block and
+// matrix kernels: its sorted point, the point's offsets and its footprint
+// start relative to the tile origin (raw, mod nf).
+struct BlkPos {
+  float off[3];
+  int st[3];
+  int p;
+};
+
+__device__ __forceinline__ void load_blk_pos(const SpreadArgs& a, int p, const int* org,
+                                             BlkPos& r) {
+  r.p = p;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    r.off[d] = a.g.off[d][p];
+    r.st[d] = a.g.cell[d][p] - org[d];
+  }
+}
+
+// The staged record of an entry at sp: header (touched-block mask, per axis
+// the byte offsets of each block's first lane weight), G strengths and the
+// three padded weight rows.  The strengths are loaded first so that their
+// round trip overlaps the weight evaluation.
+template <int G>
+__device__ __forceinline__ void write_blk_entry(float* sp, const BlkPos& r, const SpreadArgs& a,
+                                                int t0, int Gv, int WP, const int* ext,
+                                                const int* nf, const int* nb, float inv_hw,
+                                                float beta) {
+  const int W = a.g.w;
+  float2 v[G];
+#pragma unroll
+  for (int g = 0; g < G; ++g)
+    v[g] = g < Gv ? a.in[(int64_t)(t0 + g) * a.in_stride + r.p] : make_float2(0.f, 0.f);
+  uint32_t hit[3] = {0u, 0u, 0u};  // per axis: blocks along it the footprint meets
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const int st = r.st[d];
+    float* wr = sp + kBlkHead + 2 * G + d * WP;
+    for (int j = 0; j < kBlkPad; ++j) wr[j] = 0.0f;
+    for (int j = 0; j < W; ++j) wr[kBlkPad + j] = es_eval(r.off[d] - (float)j, inv_hw, beta);
+    for (int j = 0; j < kBlkPad; ++j) wr[kBlkPad + W + j] = 0.0f;
+    // offset of the block's first cell from the footprint start, in
+    // [-kBlkPad, W) when the block meets the footprint (nf >= W +
+    // kBlkPad; a block inside the grid has org + its origin < nf, so
+    // one wrap each way suffices); past the grid end: not met
+    uint32_t packed = 0u;
+    for (int bd = 0; bd < nb[d]; ++bd) {
+      const int bo = bd * kBlkE;
+      int x = bo - st;
+      if (x < 0) x += nf[d];
+      if (x >= nf[d]) x -= nf[d];
+      if (x >= nf[d] - kBlkPad) x -= nf[d];
+      if (bo < ext[d] && x < W) hit[d] |= 1u << bd;
+      const int o = kBlkHead + 2 * G + d * WP + kBlkPad + min(x, W - 1);  // < 256
+      packed |= (uint32_t)o << (8 * bd);
+    }
+    reinterpret_cast<uint32_t*>(sp)[1 + d] = packed;
+  }
+  uint32_t mask = 0u;
+  for (int b0 = 0; b0 < nb[0]; ++b0)
+    for (int b1 = 0; b1 < nb[1]; ++b1)
+      for (int b2 = 0; b2 < nb[2]; ++b2)
+        if ((hit[0] >> b0) & (hit[1] >> b1) & (hit[2] >> b2) & 1u)
+          mask |= 1u << ((b0 * nb[1] + b1) * nb[2] + b2);
+  reinterpret_cast<uint32_t*>(sp)[0] = mask;
+  float* sv = sp + kBlkHead;
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    sv[2 * g] = v[g].x;
+    sv[2 * g + 1] = v[g].y;
+  }
+}
+
+// Record of an entry for the matrix-core kernel at sp: touched-block mask,
+// G strengths and, per axis, the weight of every tile cell along it (zero
+// outside the footprint and past the grid end).  A lane's weights for a
+// block then sit at fixed offsets from the record: no header lookup before
+// the reads.  Same weights as the block record, so the same products.
+__host__ __device__ constexpr int mfma_record_floats(int G, int tiles_sum) {
+  return (kBlkHead + 2 * G + tiles_sum + 3) & ~3;
+}
+
+template <int G>
+__device__ __forceinline__ void write_mfma_entry(float* sp, const BlkPos& r, const SpreadArgs& a,
+                                                 int t0, int Gv, const int* td, const int* ext,
+                                                 const int* nf, const int* nb, float inv_hw,
+                                                 float beta) {
+  const int W = a.g.w;
+  float2 v[G];
+#pragma unroll
+  for (int g = 0; g < G; ++g)
+    v[g] = g < Gv ? a.in[(int64_t)(t0 + g) * a.in_stride + r.p] : make_float2(0.f, 0.f);
+  uint32_t hit[3] = {0u, 0u, 0u};
+  float* wr = sp + kBlkHead + 2 * G;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    // cell x of the tile is footprint cell (x - st) mod nf, st in (-nf, nf)
+    for (int x = 0; x < td[d]; ++x) {
+      int j = x - r.st[d];
+      if (j < 0) j += nf[d];
+      if (j >= nf[d]) j -= nf[d];
+      float w = 0.0f;
+      if (x < ext[d] && j < W) {
+        w = es_eval(r.off[d] - (float)j, inv_hw, beta);
+        hit[d] |= 1u << (x / kBlkE);
+      }
+      wr[x] = w;
+    }
+    wr += td[d];
+  }
+  uint32_t mask = 0u;
+  for (int b0 = 0; b0 < nb[0]; ++b0)
+    for (int b1 = 0; b1 < nb[1]; ++b1)
+      for (int b2 = 0; b2 < nb[2]; ++b2)
+        if ((hit[0] >> b0) & (hit[1] >> b1) & (hit[2] >> b2) & 1u)
+          mask |= 1u << ((b0 * nb[1] + b1) * nb[2] + b2);
+  reinterpret_cast<uint32_t*>(sp)[0] = mask;
+  float* sv = sp + kBlkHead;
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    sv[2 * g] = v[g].x;
+    sv[2 * g + 1] = v[g].y;
+  }
+}
+
 // Stage list entries [eb, eb + n) of a tile for the block and matrix
 // kernels: thread q < n writes entry eb + q's record at smem + q * PW.
 template <int G>
@@ -310,51 +435,10 @@ __device__ __forceinline__ void stage_blk_entries(const SpreadArgs& a, float* sm
                                                   int PW, int WP, const int* org, const int* ext,
                                                   const int* nf, const int* nb, int t0, int Gv,
                                                   float inv_hw, float beta) {
-  const int W = a.g.w;
   if ((int)threadIdx.x < n) {
-    float* sp = smem + threadIdx.x * PW;
-    const int p = a.list_pts[eb + threadIdx.x];
-    uint32_t hit[3] = {0u, 0u, 0u};  // per axis: blocks along it the footprint meets
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      const float off = a.g.off[d][p];
-      const int st = a.g.cell[d][p] - org[d];  // raw start, mod nf
-      float* wr = sp + kBlkHead + 2 * G + d * WP;
-      for (int j = 0; j < kBlkPad; ++j) wr[j] = 0.0f;
-      for (int j = 0; j < W; ++j) wr[kBlkPad + j] = es_eval(off - (float)j, inv_hw, beta);
-      for (int j = 0; j < kBlkPad; ++j) wr[kBlkPad + W + j] = 0.0f;
-      // offset of the block's first cell from the footprint start, in
-      // [-kBlkPad, W) when the block meets the footprint (nf >= W +
-      // kBlkPad; a block inside the grid has org + its origin < nf, so
-      // one wrap each way suffices); past the grid end: not met
-      uint32_t packed = 0u;
-      for (int bd = 0; bd < nb[d]; ++bd) {
-        const int bo = bd * kBlkE;
-        int x = bo - st;
-        if (x < 0) x += nf[d];
-        if (x >= nf[d]) x -= nf[d];
-        if (x >= nf[d] - kBlkPad) x -= nf[d];
-        if (bo < ext[d] && x < W) hit[d] |= 1u << bd;
-        const int o = kBlkHead + 2 * G + d * WP + kBlkPad + min(x, W - 1);  // < 256
-        packed |= (uint32_t)o << (8 * bd);
-      }
-      reinterpret_cast<uint32_t*>(sp)[1 + d] = packed;
-    }
-    uint32_t mask = 0u;
-    for (int b0 = 0; b0 < nb[0]; ++b0)
-      for (int b1 = 0; b1 < nb[1]; ++b1)
-        for (int b2 = 0; b2 < nb[2]; ++b2)
-          if ((hit[0] >> b0) & (hit[1] >> b1) & (hit[2] >> b2) & 1u)
-            mask |= 1u << ((b0 * nb[1] + b1) * nb[2] + b2);
-    reinterpret_cast<uint32_t*>(sp)[0] = mask;
-    float* sv = sp + kBlkHead;
-#pragma unroll
-    for (int g = 0; g < G; ++g) {
-      float2 v = make_float2(0.f, 0.f);
-      if (g < Gv) v = a.in[(int64_t)(t0 + g) * a.in_stride + p];
-      sv[2 * g] = v.x;
-      sv[2 * g + 1] = v.y;
-    }
+    BlkPos r;
+    load_blk_pos(a, a.list_pts[eb + threadIdx.x], org, r);
+    write_blk_entry<G>(smem + threadIdx.x * PW, r, a, t0, Gv, WP, ext, nf, nb, inv_hw, beta);
   }
 }
 
@@ -507,9 +591,7 @@ __global__ __launch_bounds__(4 * 64 * GH, 4 / GH) void k_spread_mfma(SpreadArgs 
   constexpr int NWAVE = 4 * GH;          // waves per workgroup
   constexpr int BPW = kBlkPerWave / GH;  // blocks per wave (16 blocks at most)
   extern __shared__ float smem[];
-  const int W = a.g.w;
-  const int WP = W + 2 * kBlkPad;
-  const int PW = blk_record_floats(W, G);
+  const int PW = mfma_record_floats(G, a.g.tile[0] + a.g.tile[1] + a.g.tile[2]);
   const int item = a.item0 + blockIdx.x;
   const int key = a.it_tile[item];
   const int set = key / a.g.nbins;
@@ -518,7 +600,7 @@ __global__ __launch_bounds__(4 * 64 * GH, 4 / GH) void k_spread_mfma(SpreadArgs 
   const int t0 = blockIdx.y * G;
   const int tb = (set - a.k0) * a.ts;
   const int Gv = min(G, a.ts - t0);
-  const float inv_hw = 2.0f / W, beta = a.g.beta;
+  const float inv_hw = 2.0f / a.g.w, beta = a.g.beta;
 
   int tc[3], org[3], ext[3], td[3], nf[3], nb[3];
   tile_coords<3>(tile, a.g.nt, tc);
@@ -560,9 +642,32 @@ __global__ __launch_bounds__(4 * 64 * GH, 4 / GH) void k_spread_mfma(SpreadArgs 
   constexpr int CHUNK = kBlkChunk * GH;  // staged entries per round (a multiple of 64)
   const int e0 = a.it_begin[item];
   const int e1 = min(e0 + a.smax, a.list_start[key + 1]);
+  // 8 transforms: thread q holds where entry q of the next chunk sits (its
+  // point, offsets and footprint start, loaded while this chunk runs on the
+  // matrix cores) and the list position of the chunk after it, so that
+  // staging waits only on the strengths' HBM round trip once the first
+  // chunk is in; 16 transforms stage directly
+  const int q = (int)threadIdx.x;
+  BlkPos cur;
+  int pn = -1;
+  if constexpr (GH == 1) {
+    if (e0 + q < e1) load_blk_pos(a, a.list_pts[e0 + q], org, cur);
+    if (q < CHUNK && e0 + CHUNK + q < e1) pn = a.list_pts[e0 + CHUNK + q];
+  }
   for (int eb = e0; eb < e1; eb += CHUNK) {
     const int n = min(CHUNK, e1 - eb);
-    stage_blk_entries<G>(a, smem, eb, n, PW, WP, org, ext, nf, nb, t0, Gv, inv_hw, beta);
+    if constexpr (GH == 1) {
+      BlkPos nxt;
+      if (pn >= 0) load_blk_pos(a, pn, org, nxt);
+      const int pnn = (q < CHUNK && eb + 2 * CHUNK + q < e1) ? a.list_pts[eb + 2 * CHUNK + q] : -1;
+      if (q < n) write_mfma_entry<G>(smem + q * PW, cur, a, t0, Gv, td, ext, nf, nb, inv_hw, beta);
+      if (pn >= 0) cur = nxt;
+      pn = pnn;
+    } else if (q < n) {
+      BlkPos r;
+      load_blk_pos(a, a.list_pts[eb + q], org, r);
+      write_mfma_entry<G>(smem + q * PW, r, a, t0, Gv, td, ext, nf, nb, inv_hw, beta);
+    }
     __syncthreads();
     for (int base = 0; base < n; base += 64) {
       const int qi = base + lane;
@@ -585,13 +690,14 @@ __global__ __launch_bounds__(4 * 64 * GH, 4 / GH) void k_spread_mfma(SpreadArgs 
           for (int h = 0; h < GH; ++h) bv[h] = 0.f;
           if (e >= 0) {
             const float* sp = smem + (base + e) * PW;
-            const uint32_t* hd = reinterpret_cast<const uint32_t*>(sp);
-            const int ox = (hd[1] >> (8 * bidx[bi][0])) & 255;
-            const int oy = (hd[2] >> (8 * bidx[bi][1])) & 255;
-            const int oz = (hd[3] >> (8 * bidx[bi][2])) & 255;
-            const float wyz = sp[oy + cy] * sp[oz + cz];
-#pragma unroll
-            for (int sx = 0; sx < kBlkE; ++sx) av[sx] = sp[ox + sx] * wyz;
+            const float* wx = sp + kBlkHead + 2 * G;
+            const float4 w4 = *reinterpret_cast<const float4*>(wx + kBlkE * bidx[bi][0]);
+            const float wyz = wx[td[0] + kBlkE * bidx[bi][1] + cy] *
+                              wx[td[0] + td[1] + kBlkE * bidx[bi][2] + cz];
+            av[0] = w4.x * wyz;
+            av[1] = w4.y * wyz;
+            av[2] = w4.z * wyz;
+            av[3] = w4.w * wyz;
 #pragma unroll
             for (int h = 0; h < GH; ++h) bv[h] = sp[kBlkHead + 16 * h + (lane & 15)];
           }
@@ -1363,7 +1469,8 @@ int launch_spread_sorted(const hipnufft_plan_s* p, const float2* in, SetRange r,
     // headline, so 8 by default
     const int GH = (r.ts > 8 && p->spread_group >= 16) ? 2 : 1;
     a.G = 8 * GH;
-    const size_t lds = sizeof(float) * (size_t)kBlkChunk * GH * blk_record_floats(p->w, 8 * GH);
+    const size_t lds = sizeof(float) * (size_t)kBlkChunk * GH *
+                       mfma_record_floats(8 * GH, p->tile[0] + p->tile[1] + p->tile[2]);
     if (nitems > 0) {
       const dim3 grd((unsigned)nitems, (unsigned)((r.ts + 8 * GH - 1) / (8 * GH)));
       StageScope timer(p, kStageSpread);
