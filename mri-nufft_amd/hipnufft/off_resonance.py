@@ -8,10 +8,13 @@
 with B (NK, L) temporal interpolators (sample m uses row m % NK, NK samples
 per shot) and C (L, *shape) spatial interpolators from a field-map
 factorization (``field_map.get_orc_factorization``) or given directly.  The
-reference runs L separate NUFFTs; here the L interpolators are L extra
-transforms per coil of ONE batched SENSE chain (``hipnufft_orc_op`` /
-``hipnufft_orc_adj``), the C_l products fused into the image-side FFT pass
-and the B weights fused into the k-space passes.
+reference runs L separate NUFFTs; for a hipnufft operator the L
+interpolators are L extra transforms per coil of ONE batched SENSE chain
+(``hipnufft_orc_op`` / ``hipnufft_orc_adj``), the C_l products fused into
+the image-side FFT pass and the B weights fused into the k-space passes.
+Any other operator of this package (the stacked one, spread/interp-only
+plans) is wrapped the reference's way: L calls of its ``op`` / ``adj_op`` on
+device tensors, the weights applied with torch.
 """
 
 from __future__ import annotations
@@ -38,9 +41,12 @@ class MRIFourierCorrected(FourierOperatorBase):
 
     def __init__(self, fourier_op, b0_map=None, readout_time=None, r2star_map=None, mask=None,
                  interpolator="svd"):
-        if not hasattr(fourier_op, "raw_op"):
-            raise ValueError("the off-resonance operator wraps a hipnufft operator")
+        if not hasattr(fourier_op, "device"):
+            raise ValueError("the off-resonance operator wraps an operator of this package")
         self._fourier_op = fourier_op
+        # the batched chain needs a full NUFFT plan of MRIHipNUFFT
+        self._fused = (hasattr(fourier_op, "raw_op")
+                       and not fourier_op._plan_opts.get("spreadinterponly"))
         self.squeeze_dims = fourier_op.squeeze_dims
         self._grad_wrt_data = False
         self._grad_wrt_traj = False
@@ -177,8 +183,6 @@ class MRIFourierCorrected(FourierOperatorBase):
         """A plan whose workspace holds as many of the B*C*L transforms as the
         budget allows (the wrapped operator's when it is large enough)."""
         fo = self._fourier_op
-        if fo._plan_opts.get("spreadinterponly"):
-            raise ValueError("the off-resonance operator needs a full NUFFT operator")
         need = fo.n_batchs * fo.n_coils * self.n_interpolators
         nf = [2 * s for s in fo.shape]
         grid = 8 * int(np.prod(nf)) * max(1, int(fo._plan_opts["upsampfac"] / 2.0) ** 2)
@@ -194,12 +198,48 @@ class MRIFourierCorrected(FourierOperatorBase):
         return self._orc_plan
 
     # --------------------------------------------------------- transforms
+    def _loop_op(self, x):
+        """``sum_l B[:, l] * A(C_l x)`` by L calls of the wrapped operator."""
+        fo = self._fourier_op
+        B, C, M = fo.n_batchs, fo.n_coils, fo.n_samples
+        NK = self.n_samples_per_shot
+        sq, fo.squeeze_dims = fo.squeeze_dims, False
+        try:
+            y = torch.zeros((B, C, M // NK, NK), dtype=torch.complex64, device=fo.device)
+            for l in range(self.n_interpolators):
+                yl = to_device(fo.op(self.C[l] * x), torch.complex64, fo.device)
+                y += yl.reshape(B, C, M // NK, NK) * self.B[:, l]
+        finally:
+            fo.squeeze_dims = sq
+        return y.reshape(B, C, M)
+
+    def _loop_adj(self, y, ic):
+        """``sum_l conj(C_l) A^H(conj(B[:, l]) * y)`` by L calls of the wrapped operator."""
+        fo = self._fourier_op
+        B, C, M = fo.n_batchs, fo.n_coils, fo.n_samples
+        NK = self.n_samples_per_shot
+        y = y.reshape(B, C, M // NK, NK)
+        sq, fo.squeeze_dims = fo.squeeze_dims, False
+        try:
+            img = torch.zeros((B, ic, *fo.shape), dtype=torch.complex64, device=fo.device)
+            for l in range(self.n_interpolators):
+                xl = fo.adj_op((y * self.B[:, l].conj()).reshape(B, C, M))
+                img += to_device(xl, torch.complex64, fo.device).reshape(img.shape) * self.C[l].conj()
+        finally:
+            fo.squeeze_dims = sq
+        return img
+
     def op(self, data, *args):
         """Forward with off-resonance -> (B, C, M) (squeezed if asked)."""
         fo = self._fourier_op
         fo.check_shape(image=data)
         kind = kind_of(data)
         B, C, M = fo.n_batchs, fo.n_coils, fo.n_samples
+        if not self._fused:
+            with torch.cuda.device(fo.device):
+                x = to_device(data, torch.complex64, fo.device)
+                x = x.reshape(B, 1 if fo.uses_sense else C, *fo.shape)
+                return from_device(self._safe_squeeze(self._loop_op(x)), kind)
         with torch.cuda.device(fo.device):
             x = fo._img_in(data, 1 if fo.uses_sense else C)
             out = torch.empty((B, C, M), dtype=torch.complex64, device=fo.device)
@@ -215,6 +255,8 @@ class MRIFourierCorrected(FourierOperatorBase):
         with torch.cuda.device(fo.device):
             y = to_device(coeffs, torch.complex64, fo.device).reshape(B, C, M)
             ic = 1 if fo.uses_sense else C
+            if not self._fused:
+                return from_device(self._safe_squeeze(self._loop_adj(y, ic)), kind)
             out = torch.empty((B, ic, *fo.shape), dtype=torch.complex64, device=fo.device)
             self._plan().orc_adj(y, fo.smaps, self.C, self.B, fo.density, out, B, C,
                                  fo.inv_norm_factor)
@@ -241,7 +283,17 @@ class MRIFourierCorrected(FourierOperatorBase):
         x = 1j * rng.random(fo.shape) + rng.random(fo.shape)
         with torch.cuda.device(fo.device):
             x = torch.from_numpy(x.astype(np.complex64)).to(fo.device)
-            lipschitz_cst, _ = power_method(max_iter, self._single_coil_AHA, x=x)
+            if self._fused:
+                lipschitz_cst, _ = power_method(max_iter, self._single_coil_AHA, x=x)
+                return float(lipschitz_cst)
+            saved = (fo._smaps, fo.n_coils, fo.n_batchs)
+            fo._smaps, fo.n_coils, fo.n_batchs = None, 1, 1
+            try:
+                lipschitz_cst, _ = power_method(
+                    max_iter, lambda v: self._loop_adj(self._loop_op(v), 1).reshape(fo.shape),
+                    x=x)
+            finally:
+                fo._smaps, fo.n_coils, fo.n_batchs = saved
         return float(lipschitz_cst)
 
     def make_autograd(self, *, wrt_data=True, wrt_traj=False, wrt_field_map=False,

@@ -225,3 +225,32 @@ def test_autograd_data_and_field_map():
     assert rel_l2(fm_grad.reshape(ref.shape), ref) < 1e-5
     with pytest.raises(ValueError):
         op.make_autograd(wrt_traj=True)
+
+
+def test_stacked_operator_wrapped_by_interpolator_loop():
+    """A stacked operator is wrapped the reference's way (L calls of its op /
+    adj_op): against the stacked oracle composed with the B / C weights."""
+    from hipnufft import MRIStackedHipNUFFT
+    from oracle import stacked as ost
+
+    shape, NS, L, nc = (12, 10, 8), 60, 3, 2
+    s2 = truncnorm_traj(NS, 2, 30)
+    zi = np.array([0, 2, 3, 5, 7])
+    M = NS * len(zi)
+    smaps = gauss_smaps(nc, shape, 31)
+    st = MRIStackedHipNUFFT(s2, shape, n_coils=nc, smaps=smaps, z_index=zi)
+    B = rand_cplx((NS, L), 32)  # one row per sample of a plane: NK = NS, a shot per plane
+    C = rand_cplx((L, *shape), 33)
+    op = st.with_off_resonance_correction(None, interpolator=(B, C))
+    assert not op._fused and op.n_shots == len(zi)
+    x = rand_cplx(shape, 34)
+    Bf = B[np.arange(M) % NS]
+    ref = sum(Bf[:, l] * ost.op(s2, zi, (C[l] * x)[None], shape, smaps) for l in range(L))
+    got = np.asarray(op.op(x))
+    assert rel_l2(got.reshape(ref.shape), ref) < 1e-5
+    y = rand_cplx((nc, M), 35)
+    ref = sum(np.conj(C[l]) * ost.adj(s2, zi, (np.conj(Bf[:, l]) * y)[None], shape, smaps)
+              for l in range(L))
+    got = np.asarray(op.adj_op(y))
+    assert rel_l2(got.reshape(ref.shape), ref) < 1e-5
+    assert op.get_lipschitz_cst(max_iter=20) > 0
