@@ -125,18 +125,21 @@ std::vector<float> deconv(int64_t M, int64_t nf, int w, double beta) {
 
 // lines per FFT workgroup: n * L elements held 16 per thread, plus the
 // twiddle table, within 64 KiB of LDS
-int lines_per_wg(int64_t n) {
+int lines_per_wg(int64_t n, int max_elems) {
   int L = 16;
-  while (L > 1 && (n * L > kMaxElemsFFT || (size_t)n * (L + 2) * sizeof(float2) > 65536)) L >>= 1;
+  while (L > 1 && (n * L > max_elems || (size_t)n * (L + 2) * sizeof(float2) > 65536)) L >>= 1;
   return L;
 }
 
 // allow_direct: a length with other prime factors gets one direct DFT pass
 // (O(n^2) per line; the z axis of stacked transforms, whose length is the
 // image's)
-int make_axis(int64_t n, FFTAxis* ax, size_t* bytes, bool allow_direct = false) {
+// strided: an axis transformed by k_fft_strided, which also runs 512-thread
+// workgroups, so its lines per workgroup may hold 2 * kMaxElemsFFT elements
+int make_axis(int64_t n, FFTAxis* ax, size_t* bytes, bool allow_direct = false,
+              bool strided = false) {
   ax->n = (int)n;
-  ax->L = lines_per_wg(n);
+  ax->L = lines_per_wg(n, strided ? 2 * kMaxElemsFFT : kMaxElemsFFT);
   if ((size_t)n * (ax->L + 2) * sizeof(float2) > 65536 || n > kMaxElemsFFT)
     return set_error(HIPNUFFT_ERR_ARG, "fine grid axis too long for the LDS FFT (max 2730)");
   int64_t m = n;
@@ -395,7 +398,8 @@ int create_fft_plan(int ndim, const int64_t* nmodes, const int64_t* nf, int ntra
       rc = set_error(HIPNUFFT_ERR_ALLOC, "cannot allocate deconvolution tables");
     p->bytes += sizeof(float) * dv.size();
   }
-  for (int d = off; d < 3 && !rc; ++d) rc = make_axis(p->nf[d], &p->fft[d], &p->bytes);
+  for (int d = off; d < 3 && !rc; ++d)
+    rc = make_axis(p->nf[d], &p->fft[d], &p->bytes, false, d < 2);
   while (!rc && p->fft[2].L > 1 && rows_adj_lds(p->fft[2], p->N[2]) > 65536) p->fft[2].L >>= 1;
   if (!rc && rows_adj_lds(p->fft[2], p->N[2]) > 65536)
     rc = set_error(HIPNUFFT_ERR_ARG, "grid too long along the last axis (max 2048)");
@@ -549,7 +553,8 @@ int hipnufft_plan_create(int type, int ndim, const int64_t* nmodes, int iflag, i
     p->bytes += sizeof(float) * dv.size();
   }
   if (!rc && !p->sio) {
-    for (int d = off; d < 3 && !rc; ++d) rc = make_axis(p->nf[d], &p->fft[d], &p->bytes);
+    for (int d = off; d < 3 && !rc; ++d)
+    rc = make_axis(p->nf[d], &p->fft[d], &p->bytes, false, d < 2);
     // the fastest axis' adjoint kernel also holds its coil sum in LDS
     while (!rc && p->fft[2].L > 1 && rows_adj_lds(p->fft[2], p->N[2]) > 65536) p->fft[2].L >>= 1;
     if (!rc && rows_adj_lds(p->fft[2], p->N[2]) > 65536)

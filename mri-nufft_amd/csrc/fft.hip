@@ -137,7 +137,7 @@ __device__ __forceinline__ void dftR(float2* v) {
 // One Stockham radix-R pass over L interleaved lines of length n held in LDS
 // at buf[pos * LP + line].  Every thread first reads all its butterflies into
 // registers, then (after a barrier) writes the results: a single LDS buffer.
-template <int R, int S, int EPT>
+template <int R, int S, int EPT, int NT = kThreads>
 __device__ __forceinline__ void stockham_pass(float2* buf, int n, int lgL, int LP, int Ns,
                                               uint32_t ns_magic, const float2* tw) {
   constexpr int MAXB = (EPT + R - 1) / R;
@@ -153,7 +153,7 @@ __device__ __forceinline__ void stockham_pass(float2* buf, int n, int lgL, int L
   float2 v[MAXB][R];
 #pragma unroll
   for (int i = 0; i < MAXB; ++i) {
-    const int b = threadIdx.x + i * kThreads;
+    const int b = threadIdx.x + i * NT;
     if (b < nbf) {
       const int l = b & (L - 1), j = b >> lgL;
       const int a0 = __mul24(j, LP) + l;
@@ -164,7 +164,7 @@ __device__ __forceinline__ void stockham_pass(float2* buf, int n, int lgL, int L
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < MAXB; ++i) {
-    const int b = threadIdx.x + i * kThreads;
+    const int b = threadIdx.x + i * NT;
     if (b < nbf) {
       const int l = b & (L - 1), j = b >> lgL;
       // q = j / Ns by multiply-high (exact for j < 2^16); Ns = 1 in the first pass
@@ -190,28 +190,29 @@ __device__ __forceinline__ void stockham_pass(float2* buf, int n, int lgL, int L
 
 // All passes of the FFT of the L lines in buf; twl is the twiddle table
 // staged in LDS by stage_twiddles (visible after the caller's barrier).
-template <int S, int EPT>
+template <int S, int EPT, int NT = kThreads>
 __device__ void fft_lines(float2* buf, const float2* twl, const FFTAxis& ax, int lgL, int LP) {
   int Ns = 1;
   for (int p = 0; p < ax.nrad; ++p) {
     const int R = ax.rad[p];
     const uint32_t mg = ax.ns_magic[p];
     switch (R) {
-      case 2: stockham_pass<2, S, EPT>(buf, ax.n, lgL, LP, Ns, mg, twl); break;
-      case 3: stockham_pass<3, S, EPT>(buf, ax.n, lgL, LP, Ns, mg, twl); break;
-      case 4: stockham_pass<4, S, EPT>(buf, ax.n, lgL, LP, Ns, mg, twl); break;
-      case 5: stockham_pass<5, S, EPT>(buf, ax.n, lgL, LP, Ns, mg, twl); break;
-      case 6: stockham_pass<6, S, EPT>(buf, ax.n, lgL, LP, Ns, mg, twl); break;
-      default: stockham_pass<8, S, EPT>(buf, ax.n, lgL, LP, Ns, mg, twl); break;
+      case 2: stockham_pass<2, S, EPT, NT>(buf, ax.n, lgL, LP, Ns, mg, twl); break;
+      case 3: stockham_pass<3, S, EPT, NT>(buf, ax.n, lgL, LP, Ns, mg, twl); break;
+      case 4: stockham_pass<4, S, EPT, NT>(buf, ax.n, lgL, LP, Ns, mg, twl); break;
+      case 5: stockham_pass<5, S, EPT, NT>(buf, ax.n, lgL, LP, Ns, mg, twl); break;
+      case 6: stockham_pass<6, S, EPT, NT>(buf, ax.n, lgL, LP, Ns, mg, twl); break;
+      default: stockham_pass<8, S, EPT, NT>(buf, ax.n, lgL, LP, Ns, mg, twl); break;
     }
     Ns *= R;
   }
 }
 
 // twiddles go after the n * LP data elements of the dynamic LDS buffer
+template <int NT = kThreads>
 __device__ __forceinline__ float2* stage_twiddles(float2* buf, const FFTAxis& ax, int LP) {
   float2* twl = buf + ax.n * LP;
-  for (int i = threadIdx.x; i < ax.n; i += kThreads) twl[i] = ax.tw[i];
+  for (int i = threadIdx.x; i < ax.n; i += NT) twl[i] = ax.tw[i];
   return twl;
 }
 
@@ -609,11 +610,11 @@ struct StridedArgs {
 // In-place FFT of L adjacent lines along a strided axis.  CONV: forward FFT,
 // multiply by the spectrum K, inverse FFT, all in LDS (the middle of the
 // Toeplitz convolution, one grid round trip instead of three).
-template <int S, bool CONV, int EPT>
-__global__ __launch_bounds__(kThreads) void k_fft_strided(StridedArgs a) {
+template <int S, bool CONV, int EPT, int NT>
+__global__ __launch_bounds__(NT) void k_fft_strided(StridedArgs a) {
   extern __shared__ float2 buf[];
   const int L = 1 << a.lgL, LP = L + 1;
-  const float2* twl = stage_twiddles(buf, a.ax, LP);
+  const float2* twl = stage_twiddles<NT>(buf, a.ax, LP);
   const BlockIdx3 bi = remapped_block();
   const int x0 = bi.x * L;
   const int oi = bi.y;
@@ -621,7 +622,7 @@ __global__ __launch_bounds__(kThreads) void k_fft_strided(StridedArgs a) {
   const int opos = a.outer_M ? kept_pos(oi, a.outer_M, a.outer_nf) : oi;
   // thread -> (line l, positions e0 + i*E); a grid holds < 2^31 cells, so
   // position offsets are 32-bit and advance by uniform steps per slot
-  const int E = kThreads >> a.lgL;
+  const int E = NT >> a.lgL;
   const int l = threadIdx.x & (L - 1), e0 = threadIdx.x >> a.lgL;
   const bool lok = x0 + l < a.nx;
   float2* base = a.data + (int64_t)t * a.tstride + (int64_t)opos * a.ostride + x0 + l;
@@ -632,7 +633,7 @@ __global__ __launch_bounds__(kThreads) void k_fft_strided(StridedArgs a) {
   const int hin = a.in_M ? a.in_M / 2 : 0;  // kept position of e: e - h (+ n if e < h)
   const int gofs = (e0 - hin) * ls, lofs = (e0 - hin) * LP + l;
   // all of a thread's loads are issued before the first LDS store, so they
-  // overlap instead of paying one memory latency each (n * L <= EPT * kThreads)
+  // overlap instead of paying one memory latency each (n * L <= EPT * NT)
   float2 v[EPT];
 #pragma unroll
   for (int i = 0; i < EPT; ++i) {
@@ -654,7 +655,7 @@ __global__ __launch_bounds__(kThreads) void k_fft_strided(StridedArgs a) {
     }
   }
   __syncthreads();
-  fft_lines<S, EPT>(buf, twl, a.ax, a.lgL, LP);
+  fft_lines<S, EPT, NT>(buf, twl, a.ax, a.lgL, LP);
   if constexpr (CONV) {
     const float2* kb = a.K + (int64_t)opos * a.ostride + x0 + l;
     const int kofs = e0 * ls;
@@ -673,7 +674,7 @@ __global__ __launch_bounds__(kThreads) void k_fft_strided(StridedArgs a) {
       }
     }
     __syncthreads();
-    fft_lines<-S, EPT>(buf, twl, a.ax, a.lgL, LP);
+    fft_lines<-S, EPT, NT>(buf, twl, a.ax, a.lgL, LP);
   }
   const int nout = a.out_M ? a.out_M : n;
   const int hout = a.out_M ? a.out_M / 2 : 0;
@@ -698,10 +699,10 @@ int lg2(int x) {
 size_t fft_lds(const FFTAxis& ax) { return sizeof(float2) * (size_t)ax.n * (ax.L + 2); }
 
 // complex elements per thread for the L lines of n elements of an axis:
-// the smallest of 8, 12, 16 with n * L <= EPT * kThreads (fewer registers,
+// the smallest of 8, 12, 16 with n * L <= EPT * nt (fewer registers,
 // more resident waves for the shorter lines)
-int ept_of(const FFTAxis& ax) {
-  const int need = (ax.n * ax.L + kThreads - 1) / kThreads;
+int ept_of(const FFTAxis& ax, int nt = kThreads) {
+  const int need = (ax.n * ax.L + nt - 1) / nt;
   return need <= 8 ? 8 : need <= 12 ? 12 : 16;
 }
 
@@ -748,18 +749,40 @@ int strided(const hipnufft_plan_s* p, int axis, int T, int in_M, int out_M, int 
   const dim3 grd((unsigned)((a.nx + ax.L - 1) / ax.L), (unsigned)a.nouter, (unsigned)T);
   StageScope timer(p, stage);
   const size_t lds = fft_lds(ax);
-  HN_EPT_SWITCH(ept_of(ax), {
-    if (K) {
-      if (sign < 0)
-        hipLaunchKernelGGL((k_fft_strided<-1, true, EE>), grd, dim3(kThreads), lds, p->stream, a);
-      else
-        hipLaunchKernelGGL((k_fft_strided<1, true, EE>), grd, dim3(kThreads), lds, p->stream, a);
-    } else if (sign < 0) {
-      hipLaunchKernelGGL((k_fft_strided<-1, false, EE>), grd, dim3(kThreads), lds, p->stream, a);
-    } else {
-      hipLaunchKernelGGL((k_fft_strided<1, false, EE>), grd, dim3(kThreads), lds, p->stream, a);
-    }
-  });
+  // strided axes may hold up to 2 * kMaxElemsFFT elements (16 lines of up to
+  // 512): those run 512-thread workgroups
+  if (ax.n * ax.L > kMaxElemsFFT) {
+    HN_EPT_SWITCH(ept_of(ax, 2 * kThreads), {
+      constexpr int NT2 = 2 * kThreads;
+      if (K) {
+        if (sign < 0)
+          hipLaunchKernelGGL((k_fft_strided<-1, true, EE, NT2>), grd, dim3(NT2), lds, p->stream, a);
+        else
+          hipLaunchKernelGGL((k_fft_strided<1, true, EE, NT2>), grd, dim3(NT2), lds, p->stream, a);
+      } else if (sign < 0) {
+        hipLaunchKernelGGL((k_fft_strided<-1, false, EE, NT2>), grd, dim3(NT2), lds, p->stream, a);
+      } else {
+        hipLaunchKernelGGL((k_fft_strided<1, false, EE, NT2>), grd, dim3(NT2), lds, p->stream, a);
+      }
+    });
+  } else {
+    HN_EPT_SWITCH(ept_of(ax), {
+      if (K) {
+        if (sign < 0)
+          hipLaunchKernelGGL((k_fft_strided<-1, true, EE, kThreads>), grd, dim3(kThreads), lds,
+                             p->stream, a);
+        else
+          hipLaunchKernelGGL((k_fft_strided<1, true, EE, kThreads>), grd, dim3(kThreads), lds,
+                             p->stream, a);
+      } else if (sign < 0) {
+        hipLaunchKernelGGL((k_fft_strided<-1, false, EE, kThreads>), grd, dim3(kThreads), lds,
+                           p->stream, a);
+      } else {
+        hipLaunchKernelGGL((k_fft_strided<1, false, EE, kThreads>), grd, dim3(kThreads), lds,
+                           p->stream, a);
+      }
+    });
+  }
   HN_LAUNCH_CHECK();
   return 0;
 }
