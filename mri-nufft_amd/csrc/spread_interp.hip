@@ -642,28 +642,10 @@ __global__ __launch_bounds__(4 * 64 * GH, 4 / GH) void k_spread_mfma(SpreadArgs 
   constexpr int CHUNK = kBlkChunk * GH;  // staged entries per round (a multiple of 64)
   const int e0 = a.it_begin[item];
   const int e1 = min(e0 + a.smax, a.list_start[key + 1]);
-  // 8 transforms: thread q holds where entry q of the next chunk sits (its
-  // point, offsets and footprint start, loaded while this chunk runs on the
-  // matrix cores) and the list position of the chunk after it, so that
-  // staging waits only on the strengths' HBM round trip once the first
-  // chunk is in; 16 transforms stage directly
   const int q = (int)threadIdx.x;
-  BlkPos cur;
-  int pn = -1;
-  if constexpr (GH == 1) {
-    if (e0 + q < e1) load_blk_pos(a, a.list_pts[e0 + q], org, cur);
-    if (q < CHUNK && e0 + CHUNK + q < e1) pn = a.list_pts[e0 + CHUNK + q];
-  }
   for (int eb = e0; eb < e1; eb += CHUNK) {
     const int n = min(CHUNK, e1 - eb);
-    if constexpr (GH == 1) {
-      BlkPos nxt;
-      if (pn >= 0) load_blk_pos(a, pn, org, nxt);
-      const int pnn = (q < CHUNK && eb + 2 * CHUNK + q < e1) ? a.list_pts[eb + 2 * CHUNK + q] : -1;
-      if (q < n) write_mfma_entry<G>(smem + q * PW, cur, a, t0, Gv, td, ext, nf, nb, inv_hw, beta);
-      if (pn >= 0) cur = nxt;
-      pn = pnn;
-    } else if (q < n) {
+    if (q < n) {
       BlkPos r;
       load_blk_pos(a, a.list_pts[eb + q], org, r);
       write_mfma_entry<G>(smem + q * PW, r, a, t0, Gv, td, ext, nf, nb, inv_hw, beta);
