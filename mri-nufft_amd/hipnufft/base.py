@@ -357,6 +357,20 @@ class FourierOperatorBase:
                 f"  n_samples: {self.n_samples}\n  uses_sense: {self.uses_sense}\n)")
 
 
+def random_start(shape, device, seed=None):
+    """``1j * U[0, 1) + U[0, 1)`` complex64 start vector of the power method,
+    drawn on ``device`` (the reference draws the same distribution with numpy
+    on the host, ``base.py:1155-1221``); seeded when ``seed`` is given."""
+    gen = torch.Generator(device=device)
+    if seed is None:
+        gen.seed()
+    else:
+        gen.manual_seed(int(seed))
+    im = torch.rand(tuple(shape), generator=gen, device=device, dtype=torch.float32)
+    re = torch.rand(tuple(shape), generator=gen, device=device, dtype=torch.float32)
+    return torch.complex(re, im)
+
+
 def power_method(max_iter, operator, norm_func=None, x=None):
     """Largest eigenvalue of A^H A (``base.py:1155-1221``).
 

@@ -28,6 +28,7 @@ import torch
 
 from . import _capi
 from ._array import from_device, kind_of, proper_trajectory, to_device
+from .base import random_start
 from .operator import DEFAULT_WORKSPACE_BYTES, MRIHipNUFFT
 
 
@@ -75,6 +76,7 @@ class MultiTrajectoryOperator:
             self.plan = _capi.RawPlan(self.shape, n_trans=self.n_trans, eps=self.eps, isign=-1,
                                       **plan_opts)
             self.plan.setpts_sets(self._coords)
+            random_start((1,), self.device, 0)  # device RNG loaded before the first solve
             self._smaps = None
             if smaps is not None:
                 s = to_device(smaps, torch.complex64, self.device)
@@ -200,10 +202,8 @@ class MultiTrajectoryOperator:
         ``cufinufft.py:1239-1279``); returns K floats.  Problems whose norm
         has converged keep their value while the others iterate."""
         K = self.n_sets
-        rng = np.random.default_rng(seed)
-        x0 = (rng.random((K, *self.shape)) + 1j * rng.random((K, *self.shape))).astype(np.complex64)
         with torch.cuda.device(self.device):
-            x = torch.from_numpy(x0).to(self.device)
+            x = random_start((K, *self.shape), self.device, seed)
             nrm = torch.linalg.vector_norm(x.reshape(K, -1), dim=1)
             x = x / nrm.reshape(K, *([1] * self.ndim))
             done = torch.zeros(K, dtype=torch.bool, device=self.device)

@@ -24,7 +24,7 @@ import torch
 
 from . import _capi
 from ._array import from_device, kind_of, to_device
-from .base import FourierOperatorBase, power_method
+from .base import FourierOperatorBase, power_method, random_start
 from .field_map import _CLazy, get_complex_fieldmap_rad, get_orc_factorization
 from .operator import DEFAULT_WORKSPACE_BYTES
 
@@ -279,10 +279,8 @@ class MRIFourierCorrected(FourierOperatorBase):
     def get_lipschitz_cst(self, max_iter=10, **kwargs):
         """Largest eigenvalue of A^H A for one coil without smaps (``off_resonance.py:312-365``)."""
         fo = self._fourier_op
-        rng = np.random.default_rng(kwargs.get("seed"))
-        x = 1j * rng.random(fo.shape) + rng.random(fo.shape)
         with torch.cuda.device(fo.device):
-            x = torch.from_numpy(x.astype(np.complex64)).to(fo.device)
+            x = random_start(fo.shape, fo.device, kwargs.get("seed"))
             if self._fused:
                 lipschitz_cst, _ = power_method(max_iter, self._single_coil_AHA, x=x)
                 return float(lipschitz_cst)

@@ -25,7 +25,7 @@ import torch
 
 from . import _capi
 from ._array import from_device, kind_of, proper_trajectory, to_device
-from .base import FourierOperatorBase, _ToggleGradPlanMixin, power_method
+from .base import FourierOperatorBase, _ToggleGradPlanMixin, power_method, random_start
 
 DEFAULT_WORKSPACE_BYTES = 24 << 30  # fine-grid workspace budget per operator
 
@@ -145,6 +145,9 @@ class MRIHipNUFFT(FourierOperatorBase, _ToggleGradPlanMixin):
                 self.compute_density(density)
             self.compute_smaps(smaps)
             self._check_n_trans()
+            # load the device RNG of the power method's start vector now, not
+            # inside the first reconstruction (its first use costs ~0.2 s)
+            random_start((1,), self.device, 0)
 
     # ------------------------------------------------------------------ setup
     def _choose_n_trans(self, n_trans):
@@ -328,19 +331,16 @@ class MRIHipNUFFT(FourierOperatorBase, _ToggleGradPlanMixin):
     def get_lipschitz_cst(self, max_iter=10, seed=None, **kwargs):
         """Power method on A^H D A without coils (``cufinufft.py:1239-1279``).
 
-        The start vector is unseeded random as in the reference; ``seed``
-        makes it reproducible (used by the serving loop so that a request
-        gives the same result alone and among others)."""
+        The start vector is unseeded random as in the reference, drawn on the
+        device; ``seed`` makes it reproducible (used by the serving loop so
+        that a request gives the same result alone and among others)."""
         n_coils, n_batchs, smaps, sq = self.n_coils, self.n_batchs, self._smaps, self.squeeze_dims
         self._smaps = None
         self.n_coils = 1
         self.n_batchs = 1
         self.squeeze_dims = True
-        rnd = np.random.random if seed is None else np.random.default_rng(seed).random
         try:
-            x = 1j * rnd(self.shape).astype(self.cpx_dtype, copy=False)
-            x += rnd(self.shape).astype(self.cpx_dtype, copy=False)
-            xt = torch.from_numpy(x).to(self.device)
+            xt = random_start(self.shape, self.device, seed)
             L, _ = power_method(max_iter, self,
                                 norm_func=lambda v: torch.linalg.vector_norm(v.reshape(-1)), x=xt)
         finally:
