@@ -893,9 +893,24 @@ __global__ __launch_bounds__(kThreads) void k_spread_reduce(SpreadArgs a) {
       ok = ok && l[d] < ext[d];
     }
     if (!ok) continue;
+    // the slots' loads are issued eight at a time (a dense centre tile has
+    // up to ~80 partials per cell); the sum keeps the slot order
+    const float2* src = a.part + ((int64_t)slot0 * a.ntrans + t) * TC + r;
+    const int64_t sstep = (int64_t)a.ntrans * TC;
     float2 sum = make_float2(0.f, 0.f);
-    for (int q = 0; q < cnt; ++q) {
-      const float2 v = a.part[((int64_t)(slot0 + q) * a.ntrans + t) * TC + r];
+    int q = 0;
+    for (; q + 8 <= cnt; q += 8) {
+      float2 v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = src[(int64_t)(q + u) * sstep];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        sum.x += v[u].x;
+        sum.y += v[u].y;
+      }
+    }
+    for (; q < cnt; ++q) {
+      const float2 v = src[(int64_t)q * sstep];
       sum.x += v.x;
       sum.y += v.y;
     }
