@@ -188,6 +188,66 @@ __device__ __forceinline__ void stockham_pass(float2* buf, int n, int lgL, int L
   __syncthreads();
 }
 
+// The same Stockham pass with the line length N, the lines 2^LGL and the
+// pass's Ns known at compile time: strides fold into the LDS instruction
+// offsets, j / Ns is a shift for the power-of-two Ns of an 8 * 8 * R plan,
+// and the butterfly-range test disappears when NT divides the butterflies.
+template <int R, int S, int EPT, int NT, int N, int LGL, int NS>
+__device__ __forceinline__ void stockham_pass_ct(float2* buf, const float2* tw) {
+  constexpr int MAXB = (EPT + R - 1) / R;
+  constexpr int L = 1 << LGL, LP = L + 1;
+  constexpr int nr = N / R;
+  constexpr int nbf = nr << LGL;
+  constexpr int tstep = N / (NS * R);
+  constexpr int in_stride = nr * LP;
+  constexpr int out_stride = NS * LP;
+  constexpr int NsR = NS * R;
+  static_assert(nbf <= MAXB * NT, "butterflies exceed the threads' slots");
+  float2 v[MAXB][R];
+#pragma unroll
+  for (int i = 0; i < MAXB; ++i) {
+    const int b = threadIdx.x + i * NT;
+    if (nbf % NT == 0 || b < nbf) {
+      const int l = b & (L - 1), j = b >> LGL;
+      const int a0 = j * LP + l;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[i][r] = buf[a0 + r * in_stride];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < MAXB; ++i) {
+    const int b = threadIdx.x + i * NT;
+    if (nbf % NT == 0 || b < nbf) {
+      const int l = b & (L - 1), j = b >> LGL;
+      const int q = j / NS;
+      const int k = j - q * NS;
+      if constexpr (NS > 1) {  // the first pass has unit twiddles
+        const int kt = k * tstep;
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          float2 w = tw[r * kt];
+          if (S > 0) w.y = -w.y;
+          v[i][r] = cmul(v[i][r], w);
+        }
+      }
+      dftR<R, S>(v[i]);
+      const int o0 = (q * NsR + k) * LP + l;
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[o0 + r * out_stride] = v[i][r];
+    }
+  }
+  __syncthreads();
+}
+
+// 384 = 8 * 8 * 6 points on 2^LGL lines, every size fixed at compile time
+template <int S, int EPT, int NT, int LGL>
+__device__ __forceinline__ void fft_lines_384(float2* buf, const float2* twl) {
+  stockham_pass_ct<8, S, EPT, NT, 384, LGL, 1>(buf, twl);
+  stockham_pass_ct<8, S, EPT, NT, 384, LGL, 8>(buf, twl);
+  stockham_pass_ct<6, S, EPT, NT, 384, LGL, 64>(buf, twl);
+}
+
 // All passes of the FFT of the L lines in buf; twl is the twiddle table
 // staged in LDS by stage_twiddles (visible after the caller's barrier).
 template <int S, int EPT, int NT = kThreads>
@@ -610,9 +670,12 @@ struct StridedArgs {
 // In-place FFT of L adjacent lines along a strided axis.  CONV: forward FFT,
 // multiply by the spectrum K, inverse FFT, all in LDS (the middle of the
 // Toeplitz convolution, one grid round trip instead of three).
-template <int S, bool CONV, int EPT, int NT>
+template <int S, bool CONV, int EPT, int NT, int FIX = 0>
 __global__ __launch_bounds__(NT) void k_fft_strided(StridedArgs a) {
   extern __shared__ float2 buf[];
+  // FIX = 384: 384-point lines, 16 per workgroup, passes fixed at compile time
+  constexpr int kFixLgL = 4;
+  if constexpr (FIX == 384) a.lgL = kFixLgL;
   const int L = 1 << a.lgL, LP = L + 1;
   const float2* twl = stage_twiddles<NT>(buf, a.ax, LP);
   const BlockIdx3 bi = remapped_block();
@@ -655,7 +718,10 @@ __global__ __launch_bounds__(NT) void k_fft_strided(StridedArgs a) {
     }
   }
   __syncthreads();
-  fft_lines<S, EPT, NT>(buf, twl, a.ax, a.lgL, LP);
+  if constexpr (FIX == 384)
+    fft_lines_384<S, EPT, NT, kFixLgL>(buf, twl);
+  else
+    fft_lines<S, EPT, NT>(buf, twl, a.ax, a.lgL, LP);
   if constexpr (CONV) {
     const float2* kb = a.K + (int64_t)opos * a.ostride + x0 + l;
     const int kofs = e0 * ls;
@@ -674,7 +740,10 @@ __global__ __launch_bounds__(NT) void k_fft_strided(StridedArgs a) {
       }
     }
     __syncthreads();
-    fft_lines<-S, EPT, NT>(buf, twl, a.ax, a.lgL, LP);
+    if constexpr (FIX == 384)
+      fft_lines_384<-S, EPT, NT, kFixLgL>(buf, twl);
+    else
+      fft_lines<-S, EPT, NT>(buf, twl, a.ax, a.lgL, LP);
   }
   const int nout = a.out_M ? a.out_M : n;
   const int hout = a.out_M ? a.out_M / 2 : 0;
@@ -751,7 +820,21 @@ int strided(const hipnufft_plan_s* p, int axis, int T, int in_M, int out_M, int 
   const size_t lds = fft_lds(ax);
   // strided axes may hold up to 2 * kMaxElemsFFT elements (16 lines of up to
   // 512): those run 512-thread workgroups
-  if (ax.n * ax.L > kMaxElemsFFT) {
+  if (ax.n == 384 && ax.L == 16 && ax.nrad == 3 && ax.rad[0] == 8 && ax.rad[1] == 8 &&
+      ax.rad[2] == 6) {
+    // 384 * 16 / 512 threads = 12 elements each
+    constexpr int NT2 = 2 * kThreads;
+    if (K) {
+      if (sign < 0)
+        hipLaunchKernelGGL((k_fft_strided<-1, true, 12, NT2, 384>), grd, dim3(NT2), lds, p->stream, a);
+      else
+        hipLaunchKernelGGL((k_fft_strided<1, true, 12, NT2, 384>), grd, dim3(NT2), lds, p->stream, a);
+    } else if (sign < 0) {
+      hipLaunchKernelGGL((k_fft_strided<-1, false, 12, NT2, 384>), grd, dim3(NT2), lds, p->stream, a);
+    } else {
+      hipLaunchKernelGGL((k_fft_strided<1, false, 12, NT2, 384>), grd, dim3(NT2), lds, p->stream, a);
+    }
+  } else if (ax.n * ax.L > kMaxElemsFFT) {
     HN_EPT_SWITCH(ept_of(ax, 2 * kThreads), {
       constexpr int NT2 = 2 * kThreads;
       if (K) {
