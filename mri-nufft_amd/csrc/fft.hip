@@ -308,9 +308,12 @@ struct RowsArgs {
 };
 
 // image rows -> deconvolve * scale (* smap) -> pad -> FFT -> grid rows
-template <int S, int EPT>
+template <int S, int EPT, int FIX = 0>
 __global__ __launch_bounds__(kThreads) void k_rows_fwd(RowsArgs a) {
   extern __shared__ float2 buf[];
+  // FIX = 384: 384-point rows, 8 per workgroup, passes fixed at compile time
+  constexpr int kFixLgL = 3;
+  if constexpr (FIX == 384) a.lgL = kFixLgL;
   const int L = 1 << a.lgL, LP = L + 1;
   const float2* twl = stage_twiddles(buf, a.ax, LP);
   const int rows = a.M0 * a.M1;
@@ -366,7 +369,10 @@ __global__ __launch_bounds__(kThreads) void k_rows_fwd(RowsArgs a) {
     if (e < gap) buf[(gap0 + e) * LP + l] = make_float2(0.f, 0.f);
   }
   __syncthreads();
-  fft_lines<S, EPT>(buf, twl, a.ax, a.lgL, LP);
+  if constexpr (FIX == 384)
+    fft_lines_384<S, EPT, kThreads, kFixLgL>(buf, twl);
+  else
+    fft_lines<S, EPT>(buf, twl, a.ax, a.lgL, LP);
   if (!rok) return;
   float2* g = a.grid + (int64_t)t * a.gstride +
               ((int64_t)kept_pos(i0, a.M0, a.nf0) * a.nf1 + kept_pos(i1, a.M1, a.nf1)) * n;
@@ -379,9 +385,12 @@ __global__ __launch_bounds__(kThreads) void k_rows_fwd(RowsArgs a) {
 
 // grid rows -> FFT -> crop -> deconvolve * scale (* conj smap, summed over the
 // chunk's coils of one batch) -> image rows
-template <int S, int EPT>
+template <int S, int EPT, int FIX = 0>
 __global__ __launch_bounds__(kThreads) void k_rows_adj(RowsArgs a) {
   extern __shared__ float2 buf[];
+  // FIX = 384: 384-point rows, 8 per workgroup, passes fixed at compile time
+  constexpr int kFixLgL = 3;
+  if constexpr (FIX == 384) a.lgL = kFixLgL;
   const int L = 1 << a.lgL, LP = L + 1;
   const float2* twl = stage_twiddles(buf, a.ax, LP);
   const int rows = a.M0 * a.M1;
@@ -430,9 +439,13 @@ __global__ __launch_bounds__(kThreads) void k_rows_adj(RowsArgs a) {
     // the FFT's addressing is the same for every coil; hiding the line
     // geometry from the optimiser keeps it from hoisting all of it out of the
     // coil loop into registers
-    int lg = a.lgL, lp = LP;
-    asm volatile("" : "+s"(lg), "+s"(lp));
-    fft_lines<S, EPT>(buf, twl, a.ax, lg, lp);
+    if constexpr (FIX == 384) {
+      fft_lines_384<S, EPT, kThreads, kFixLgL>(buf, twl);
+    } else {
+      int lg = a.lgL, lp = LP;
+      asm volatile("" : "+s"(lg), "+s"(lp));
+      fft_lines<S, EPT>(buf, twl, a.ax, lg, lp);
+    }
     const float2* sm =
         a.smaps ? a.smaps + (int64_t)(a.smaps_all ? tt : (tt / a.L) % a.C) * N + (int64_t)r * a.M2
                 : nullptr;
@@ -767,6 +780,13 @@ int lg2(int x) {
 
 size_t fft_lds(const FFTAxis& ax) { return sizeof(float2) * (size_t)ax.n * (ax.L + 2); }
 
+// the axis is 384 = 8 * 8 * 6 points on L lines: the compile-time passes
+// (fft_lines_384) apply
+bool fixed_384(const FFTAxis& ax, int L) {
+  return ax.n == 384 && ax.L == L && !ax.direct && ax.nrad == 3 && ax.rad[0] == 8 &&
+         ax.rad[1] == 8 && ax.rad[2] == 6;
+}
+
 // complex elements per thread for the L lines of n elements of an axis:
 // the smallest of 8, 12, 16 with n * L <= EPT * nt (fewer registers,
 // more resident waves for the shorter lines)
@@ -820,8 +840,7 @@ int strided(const hipnufft_plan_s* p, int axis, int T, int in_M, int out_M, int 
   const size_t lds = fft_lds(ax);
   // strided axes may hold up to 2 * kMaxElemsFFT elements (16 lines of up to
   // 512): those run 512-thread workgroups
-  if (ax.n == 384 && ax.L == 16 && ax.nrad == 3 && ax.rad[0] == 8 && ax.rad[1] == 8 &&
-      ax.rad[2] == 6) {
+  if (fixed_384(ax, 16)) {
     // 384 * 16 / 512 threads = 12 elements each
     constexpr int NT2 = 2 * kThreads;
     if (K) {
@@ -936,12 +955,19 @@ int rows_fwd(const hipnufft_plan_s* p, const float2* img, const float2* smaps, i
   const size_t lds = fft_lds(p->fft[2]);
   {
     StageScope timer(p, kStageRowsFwd);
-    HN_EPT_SWITCH(ept_of(p->fft[2]), {
+    if (fixed_384(p->fft[2], 8)) {
       if (sign < 0)
-        hipLaunchKernelGGL((k_rows_fwd<-1, EE>), grd, dim3(kThreads), lds, p->stream, a);
+        hipLaunchKernelGGL((k_rows_fwd<-1, 12, 384>), grd, dim3(kThreads), lds, p->stream, a);
       else
-        hipLaunchKernelGGL((k_rows_fwd<1, EE>), grd, dim3(kThreads), lds, p->stream, a);
-    });
+        hipLaunchKernelGGL((k_rows_fwd<1, 12, 384>), grd, dim3(kThreads), lds, p->stream, a);
+    } else {
+      HN_EPT_SWITCH(ept_of(p->fft[2]), {
+        if (sign < 0)
+          hipLaunchKernelGGL((k_rows_fwd<-1, EE>), grd, dim3(kThreads), lds, p->stream, a);
+        else
+          hipLaunchKernelGGL((k_rows_fwd<1, EE>), grd, dim3(kThreads), lds, p->stream, a);
+      });
+    }
     HN_LAUNCH_CHECK();
   }
   return 0;
@@ -966,12 +992,19 @@ int rows_adj(const hipnufft_plan_s* p, float2* img, const float2* smaps, int C, 
   const dim3 grd((unsigned)((rows + L - 1) / L), (unsigned)nimg);
   const size_t lds = rows_adj_lds(p->fft[2], a.M2);
   StageScope timer(p, kStageRowsAdj);
-  HN_EPT_SWITCH(ept_of(p->fft[2]), {
+  if (fixed_384(p->fft[2], 8)) {
     if (sign < 0)
-      hipLaunchKernelGGL((k_rows_adj<-1, EE>), grd, dim3(kThreads), lds, p->stream, a);
+      hipLaunchKernelGGL((k_rows_adj<-1, 12, 384>), grd, dim3(kThreads), lds, p->stream, a);
     else
-      hipLaunchKernelGGL((k_rows_adj<1, EE>), grd, dim3(kThreads), lds, p->stream, a);
-  });
+      hipLaunchKernelGGL((k_rows_adj<1, 12, 384>), grd, dim3(kThreads), lds, p->stream, a);
+  } else {
+    HN_EPT_SWITCH(ept_of(p->fft[2]), {
+      if (sign < 0)
+        hipLaunchKernelGGL((k_rows_adj<-1, EE>), grd, dim3(kThreads), lds, p->stream, a);
+      else
+        hipLaunchKernelGGL((k_rows_adj<1, EE>), grd, dim3(kThreads), lds, p->stream, a);
+    });
+  }
   HN_LAUNCH_CHECK();
   return 0;
 }
