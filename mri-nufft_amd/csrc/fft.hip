@@ -240,12 +240,40 @@ __device__ __forceinline__ void stockham_pass_ct(float2* buf, const float2* tw) 
   __syncthreads();
 }
 
-// 384 = 8 * 8 * 6 points on 2^LGL lines, every size fixed at compile time
-template <int S, int EPT, int NT, int LGL>
-__device__ __forceinline__ void fft_lines_384(float2* buf, const float2* twl) {
-  stockham_pass_ct<8, S, EPT, NT, 384, LGL, 1>(buf, twl);
-  stockham_pass_ct<8, S, EPT, NT, 384, LGL, 8>(buf, twl);
-  stockham_pass_ct<6, S, EPT, NT, 384, LGL, 64>(buf, twl);
+// The p-th radix of the plan of an n-point line: 8s, then at most one each of
+// 4, 6 and 2, then 3s and 5s (0 past the end) -- the rule of the host's axis
+// set-up, so that a compile-time plan can be checked against the runtime one
+__host__ __device__ constexpr int plan_radix(int n, int p) {
+  int rad[16] = {};
+  int k = 0, m = n;
+  while (m % 8 == 0 && k < 16) { rad[k++] = 8; m /= 8; }
+  if (m % 4 == 0) { rad[k++] = 4; m /= 4; }
+  if (m % 6 == 0) { rad[k++] = 6; m /= 6; }
+  if (m % 2 == 0) { rad[k++] = 2; m /= 2; }
+  while (m % 3 == 0 && k < 16) { rad[k++] = 3; m /= 3; }
+  while (m % 5 == 0 && k < 16) { rad[k++] = 5; m /= 5; }
+  return p < k ? rad[p] : 0;
+}
+
+__host__ __device__ constexpr int plan_ns(int n, int p) {
+  int ns = 1;
+  for (int i = 0; i < p; ++i) ns *= plan_radix(n, i);
+  return ns;
+}
+
+template <int S, int EPT, int NT, int N, int LGL, int P>
+__device__ __forceinline__ void fft_lines_ct_from(float2* buf, const float2* twl) {
+  constexpr int R = plan_radix(N, P);
+  if constexpr (R != 0) {
+    stockham_pass_ct<R, S, EPT, NT, N, LGL, plan_ns(N, P)>(buf, twl);
+    fft_lines_ct_from<S, EPT, NT, N, LGL, P + 1>(buf, twl);
+  }
+}
+
+// All passes of N-point lines on 2^LGL lines, every size fixed at compile time
+template <int S, int EPT, int NT, int N, int LGL>
+__device__ __forceinline__ void fft_lines_ct(float2* buf, const float2* twl) {
+  fft_lines_ct_from<S, EPT, NT, N, LGL, 0>(buf, twl);
 }
 
 // All passes of the FFT of the L lines in buf; twl is the twiddle table
@@ -308,12 +336,11 @@ struct RowsArgs {
 };
 
 // image rows -> deconvolve * scale (* smap) -> pad -> FFT -> grid rows
-template <int S, int EPT, int FIX = 0>
+template <int S, int EPT, int FIX = 0, int FLGL = 0>
 __global__ __launch_bounds__(kThreads) void k_rows_fwd(RowsArgs a) {
   extern __shared__ float2 buf[];
-  // FIX = 384: 384-point rows, 8 per workgroup, passes fixed at compile time
-  constexpr int kFixLgL = 3;
-  if constexpr (FIX == 384) a.lgL = kFixLgL;
+  // FIX > 0: FIX-point rows, 2^FLGL per workgroup, passes fixed at compile time
+  if constexpr (FIX > 0) a.lgL = FLGL;
   const int L = 1 << a.lgL, LP = L + 1;
   const float2* twl = stage_twiddles(buf, a.ax, LP);
   const int rows = a.M0 * a.M1;
@@ -369,8 +396,8 @@ __global__ __launch_bounds__(kThreads) void k_rows_fwd(RowsArgs a) {
     if (e < gap) buf[(gap0 + e) * LP + l] = make_float2(0.f, 0.f);
   }
   __syncthreads();
-  if constexpr (FIX == 384)
-    fft_lines_384<S, EPT, kThreads, kFixLgL>(buf, twl);
+  if constexpr (FIX > 0)
+    fft_lines_ct<S, EPT, kThreads, FIX, FLGL>(buf, twl);
   else
     fft_lines<S, EPT>(buf, twl, a.ax, a.lgL, LP);
   if (!rok) return;
@@ -385,12 +412,11 @@ __global__ __launch_bounds__(kThreads) void k_rows_fwd(RowsArgs a) {
 
 // grid rows -> FFT -> crop -> deconvolve * scale (* conj smap, summed over the
 // chunk's coils of one batch) -> image rows
-template <int S, int EPT, int FIX = 0>
+template <int S, int EPT, int FIX = 0, int FLGL = 0>
 __global__ __launch_bounds__(kThreads) void k_rows_adj(RowsArgs a) {
   extern __shared__ float2 buf[];
-  // FIX = 384: 384-point rows, 8 per workgroup, passes fixed at compile time
-  constexpr int kFixLgL = 3;
-  if constexpr (FIX == 384) a.lgL = kFixLgL;
+  // FIX > 0: FIX-point rows, 2^FLGL per workgroup, passes fixed at compile time
+  if constexpr (FIX > 0) a.lgL = FLGL;
   const int L = 1 << a.lgL, LP = L + 1;
   const float2* twl = stage_twiddles(buf, a.ax, LP);
   const int rows = a.M0 * a.M1;
@@ -439,8 +465,8 @@ __global__ __launch_bounds__(kThreads) void k_rows_adj(RowsArgs a) {
     // the FFT's addressing is the same for every coil; hiding the line
     // geometry from the optimiser keeps it from hoisting all of it out of the
     // coil loop into registers
-    if constexpr (FIX == 384) {
-      fft_lines_384<S, EPT, kThreads, kFixLgL>(buf, twl);
+    if constexpr (FIX > 0) {
+      fft_lines_ct<S, EPT, kThreads, FIX, FLGL>(buf, twl);
     } else {
       int lg = a.lgL, lp = LP;
       asm volatile("" : "+s"(lg), "+s"(lp));
@@ -683,12 +709,11 @@ struct StridedArgs {
 // In-place FFT of L adjacent lines along a strided axis.  CONV: forward FFT,
 // multiply by the spectrum K, inverse FFT, all in LDS (the middle of the
 // Toeplitz convolution, one grid round trip instead of three).
-template <int S, bool CONV, int EPT, int NT, int FIX = 0>
+template <int S, bool CONV, int EPT, int NT, int FIX = 0, int FLGL = 0>
 __global__ __launch_bounds__(NT) void k_fft_strided(StridedArgs a) {
   extern __shared__ float2 buf[];
-  // FIX = 384: 384-point lines, 16 per workgroup, passes fixed at compile time
-  constexpr int kFixLgL = 4;
-  if constexpr (FIX == 384) a.lgL = kFixLgL;
+  // FIX > 0: FIX-point lines, 2^FLGL per workgroup, passes fixed at compile time
+  if constexpr (FIX > 0) a.lgL = FLGL;
   const int L = 1 << a.lgL, LP = L + 1;
   const float2* twl = stage_twiddles<NT>(buf, a.ax, LP);
   const BlockIdx3 bi = remapped_block();
@@ -731,8 +756,8 @@ __global__ __launch_bounds__(NT) void k_fft_strided(StridedArgs a) {
     }
   }
   __syncthreads();
-  if constexpr (FIX == 384)
-    fft_lines_384<S, EPT, NT, kFixLgL>(buf, twl);
+  if constexpr (FIX > 0)
+    fft_lines_ct<S, EPT, NT, FIX, FLGL>(buf, twl);
   else
     fft_lines<S, EPT, NT>(buf, twl, a.ax, a.lgL, LP);
   if constexpr (CONV) {
@@ -753,8 +778,8 @@ __global__ __launch_bounds__(NT) void k_fft_strided(StridedArgs a) {
       }
     }
     __syncthreads();
-    if constexpr (FIX == 384)
-      fft_lines_384<-S, EPT, NT, kFixLgL>(buf, twl);
+    if constexpr (FIX > 0)
+      fft_lines_ct<-S, EPT, NT, FIX, FLGL>(buf, twl);
     else
       fft_lines<-S, EPT, NT>(buf, twl, a.ax, a.lgL, LP);
   }
@@ -780,11 +805,14 @@ int lg2(int x) {
 
 size_t fft_lds(const FFTAxis& ax) { return sizeof(float2) * (size_t)ax.n * (ax.L + 2); }
 
-// the axis is 384 = 8 * 8 * 6 points on L lines: the compile-time passes
-// (fft_lines_384) apply
-bool fixed_384(const FFTAxis& ax, int L) {
-  return ax.n == 384 && ax.L == L && !ax.direct && ax.nrad == 3 && ax.rad[0] == 8 &&
-         ax.rad[1] == 8 && ax.rad[2] == 6;
+// the axis has n points on L lines and the runtime radix plan is the
+// compile-time one of fft_lines_ct
+bool fixed_plan(const FFTAxis& ax, int n, int L) {
+  if (ax.n != n || ax.L != L || ax.direct) return false;
+  int p = 0;
+  for (; p < ax.nrad; ++p)
+    if (ax.rad[p] != plan_radix(n, p)) return false;
+  return plan_radix(n, p) == 0;
 }
 
 // complex elements per thread for the L lines of n elements of an axis:
@@ -840,20 +868,31 @@ int strided(const hipnufft_plan_s* p, int axis, int T, int in_M, int out_M, int 
   const size_t lds = fft_lds(ax);
   // strided axes may hold up to 2 * kMaxElemsFFT elements (16 lines of up to
   // 512): those run 512-thread workgroups
-  if (fixed_384(ax, 16)) {
-    // 384 * 16 / 512 threads = 12 elements each
-    constexpr int NT2 = 2 * kThreads;
-    if (K) {
-      if (sign < 0)
-        hipLaunchKernelGGL((k_fft_strided<-1, true, 12, NT2, 384>), grd, dim3(NT2), lds, p->stream, a);
-      else
-        hipLaunchKernelGGL((k_fft_strided<1, true, 12, NT2, 384>), grd, dim3(NT2), lds, p->stream, a);
-    } else if (sign < 0) {
-      hipLaunchKernelGGL((k_fft_strided<-1, false, 12, NT2, 384>), grd, dim3(NT2), lds, p->stream, a);
-    } else {
-      hipLaunchKernelGGL((k_fft_strided<1, false, 12, NT2, 384>), grd, dim3(NT2), lds, p->stream, a);
-    }
-  } else if (ax.n * ax.L > kMaxElemsFFT) {
+// compile-time plans: (points, log2 lines, threads, elements per thread)
+#define HN_STRIDED_FIXED(NN, LG, NTT, EPTT)                                                    \
+  if (fixed_plan(ax, NN, 1 << LG)) {                                                             \
+    if (K) {                                                                                     \
+      if (sign < 0)                                                                              \
+        hipLaunchKernelGGL((k_fft_strided<-1, true, EPTT, NTT, NN, LG>), grd, dim3(NTT), lds,    \
+                           p->stream, a);                                                        \
+      else                                                                                       \
+        hipLaunchKernelGGL((k_fft_strided<1, true, EPTT, NTT, NN, LG>), grd, dim3(NTT), lds,     \
+                           p->stream, a);                                                        \
+    } else if (sign < 0) {                                                                       \
+      hipLaunchKernelGGL((k_fft_strided<-1, false, EPTT, NTT, NN, LG>), grd, dim3(NTT), lds,     \
+                         p->stream, a);                                                          \
+    } else {                                                                                     \
+      hipLaunchKernelGGL((k_fft_strided<1, false, EPTT, NTT, NN, LG>), grd, dim3(NTT), lds,      \
+                         p->stream, a);                                                          \
+    }                                                                                            \
+    HN_LAUNCH_CHECK();                                                                           \
+    return 0;                                                                                    \
+  }
+  HN_STRIDED_FIXED(384, 4, 2 * kThreads, 12)
+  HN_STRIDED_FIXED(256, 4, kThreads, 16)
+  HN_STRIDED_FIXED(512, 3, kThreads, 16)
+#undef HN_STRIDED_FIXED
+  if (ax.n * ax.L > kMaxElemsFFT) {
     HN_EPT_SWITCH(ept_of(ax, 2 * kThreads), {
       constexpr int NT2 = 2 * kThreads;
       if (K) {
@@ -955,11 +994,22 @@ int rows_fwd(const hipnufft_plan_s* p, const float2* img, const float2* smaps, i
   const size_t lds = fft_lds(p->fft[2]);
   {
     StageScope timer(p, kStageRowsFwd);
-    if (fixed_384(p->fft[2], 8)) {
+    const FFTAxis& ax = p->fft[2];
+    if (fixed_plan(ax, 384, 8)) {
       if (sign < 0)
-        hipLaunchKernelGGL((k_rows_fwd<-1, 12, 384>), grd, dim3(kThreads), lds, p->stream, a);
+        hipLaunchKernelGGL((k_rows_fwd<-1, 12, 384, 3>), grd, dim3(kThreads), lds, p->stream, a);
       else
-        hipLaunchKernelGGL((k_rows_fwd<1, 12, 384>), grd, dim3(kThreads), lds, p->stream, a);
+        hipLaunchKernelGGL((k_rows_fwd<1, 12, 384, 3>), grd, dim3(kThreads), lds, p->stream, a);
+    } else if (fixed_plan(ax, 256, 16)) {
+      if (sign < 0)
+        hipLaunchKernelGGL((k_rows_fwd<-1, 16, 256, 4>), grd, dim3(kThreads), lds, p->stream, a);
+      else
+        hipLaunchKernelGGL((k_rows_fwd<1, 16, 256, 4>), grd, dim3(kThreads), lds, p->stream, a);
+    } else if (fixed_plan(ax, 512, 8)) {
+      if (sign < 0)
+        hipLaunchKernelGGL((k_rows_fwd<-1, 16, 512, 3>), grd, dim3(kThreads), lds, p->stream, a);
+      else
+        hipLaunchKernelGGL((k_rows_fwd<1, 16, 512, 3>), grd, dim3(kThreads), lds, p->stream, a);
     } else {
       HN_EPT_SWITCH(ept_of(p->fft[2]), {
         if (sign < 0)
@@ -992,11 +1042,22 @@ int rows_adj(const hipnufft_plan_s* p, float2* img, const float2* smaps, int C, 
   const dim3 grd((unsigned)((rows + L - 1) / L), (unsigned)nimg);
   const size_t lds = rows_adj_lds(p->fft[2], a.M2);
   StageScope timer(p, kStageRowsAdj);
-  if (fixed_384(p->fft[2], 8)) {
+  const FFTAxis& ax = p->fft[2];
+  if (fixed_plan(ax, 384, 8)) {
     if (sign < 0)
-      hipLaunchKernelGGL((k_rows_adj<-1, 12, 384>), grd, dim3(kThreads), lds, p->stream, a);
+      hipLaunchKernelGGL((k_rows_adj<-1, 12, 384, 3>), grd, dim3(kThreads), lds, p->stream, a);
     else
-      hipLaunchKernelGGL((k_rows_adj<1, 12, 384>), grd, dim3(kThreads), lds, p->stream, a);
+      hipLaunchKernelGGL((k_rows_adj<1, 12, 384, 3>), grd, dim3(kThreads), lds, p->stream, a);
+  } else if (fixed_plan(ax, 256, 16)) {
+    if (sign < 0)
+      hipLaunchKernelGGL((k_rows_adj<-1, 16, 256, 4>), grd, dim3(kThreads), lds, p->stream, a);
+    else
+      hipLaunchKernelGGL((k_rows_adj<1, 16, 256, 4>), grd, dim3(kThreads), lds, p->stream, a);
+  } else if (fixed_plan(ax, 512, 8)) {
+    if (sign < 0)
+      hipLaunchKernelGGL((k_rows_adj<-1, 16, 512, 3>), grd, dim3(kThreads), lds, p->stream, a);
+    else
+      hipLaunchKernelGGL((k_rows_adj<1, 16, 512, 3>), grd, dim3(kThreads), lds, p->stream, a);
   } else {
     HN_EPT_SWITCH(ept_of(p->fft[2]), {
       if (sign < 0)
