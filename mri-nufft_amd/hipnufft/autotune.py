@@ -150,7 +150,7 @@ def autotune(coords, shape, n_trans=1, eps=1e-6, upsampfac=2.0, reps=3, budget_s
         ("spread_group", [4, 8, 16]),
         ("interp_group", [1, 2, 4] if ndim < 3 else [1]),
         ("max_subproblem", [512, 1024, 2048]),
-        ("spread_max", [1024, 2048, 4096, 8192]),
+        ("spread_max", [256, 512, 1024, 2048, 4096]),
     ]
     for name, values in stages:
         for v in values:
