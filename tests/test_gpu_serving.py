@@ -38,7 +38,12 @@ def test_alone_vs_mix_identical(toeplitz):
     reqs = _requests()
     mix = ReconServer(max_slots=4, toeplitz=toeplitz).run(reqs, realtime=False)
     assert sorted(mix) == list(range(len(reqs)))
+    # all requests wait from the start: latencies count from the call
+    assert all(m.arrival == 0.0 and 0.0 <= m.admitted <= m.finished for m in mix.values())
+    # every iteration enqueued at admission: the same solves
+    deep = ReconServer(max_slots=4, toeplitz=toeplitz, depth=0).run(reqs, realtime=False)
     for r in reqs:
+        assert rel_l2(deep[r.rid].image, mix[r.rid].image) <= 1e-6, r.rid
         alone = ReconServer(max_slots=1, toeplitz=toeplitz).run([r], realtime=False)[r.rid]
         assert rel_l2(mix[r.rid].image, alone.image) <= 1e-6, r.rid
 
