@@ -328,12 +328,16 @@ class MRIHipNUFFT(FourierOperatorBase, _ToggleGradPlanMixin):
         return from_device(self._safe_squeeze(out), kind)
 
     # --------------------------------------------------------------- extras
-    def get_lipschitz_cst(self, max_iter=10, seed=None, **kwargs):
+    def get_lipschitz_cst(self, max_iter=10, seed=None, toeplitz=False, **kwargs):
         """Power method on A^H D A without coils (``cufinufft.py:1239-1279``).
 
         The start vector is unseeded random as in the reference, drawn on the
         device; ``seed`` makes it reproducible (used by the serving loop so
-        that a request gives the same result alone and among others)."""
+        that a request gives the same result alone and among others).
+        ``toeplitz=True`` iterates the Toeplitz form of the same operator
+        (FFTs only, built if missing): equal to the NUFFT form at the NUFFT
+        tolerance, and much cheaper for large trajectories, whose one-coil
+        spreading is bound by staging the list entries."""
         n_coils, n_batchs, smaps, sq = self.n_coils, self.n_batchs, self._smaps, self.squeeze_dims
         self._smaps = None
         self.n_coils = 1
@@ -341,7 +345,8 @@ class MRIHipNUFFT(FourierOperatorBase, _ToggleGradPlanMixin):
         self.squeeze_dims = True
         try:
             xt = random_start(self.shape, self.device, seed)
-            L, _ = power_method(max_iter, self,
+            A = self.gram_op if toeplitz else self
+            L, _ = power_method(max_iter, A,
                                 norm_func=lambda v: torch.linalg.vector_norm(v.reshape(-1)), x=xt)
         finally:
             self.n_coils = n_coils

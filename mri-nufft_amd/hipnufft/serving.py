@@ -116,10 +116,11 @@ class _Plan:
             self.op._smaps = torch.zeros((n_coils, *shape), dtype=torch.complex64,
                                          device=self.op.device)
             self.op.n_coils = n_coils
-            self.lipschitz = float(self.op.get_lipschitz_cst(seed=seed))
             self.toeplitz = bool(toeplitz) and toeplitz_supported(shape)
             if self.toeplitz:
                 self.op.compute_toeplitz_kernel()
+            # the power iterations on the Toeplitz form when there is one
+            self.lipschitz = float(self.op.get_lipschitz_cst(seed=seed, toeplitz=self.toeplitz))
         self.key = key
         self.bytes = estimate_bytes(shape, self.op.n_samples, max(n_coils, self.op.n_trans),
                                     toeplitz=self.toeplitz)

@@ -165,6 +165,19 @@ def test_update_samples_equals_fresh():
     np.testing.assert_array_equal(op.adj_op(y), fresh.adj_op(y))
 
 
+@pytest.mark.parametrize("name", ["random2D", "random3D"])
+def test_lipschitz_toeplitz_form(name):
+    """The power iterations on the Toeplitz form give the NUFFT form's value
+    from the same start vector (the two operators agree to the NUFFT
+    tolerance)."""
+    k, shape = CASES[name]()
+    op = get_op(k, shape, n_coils=2, density=np.linspace(0.5, 1.5, len(k)).astype(np.float32))
+    L = op.get_lipschitz_cst(10, seed=5)
+    Lt = op.get_lipschitz_cst(10, seed=5, toeplitz=True)
+    assert abs(Lt - L) <= 1e-4 * L
+    assert op.n_coils == 2 and op._smaps is None
+
+
 def test_lipschitz_upper_bound():
     from hipnufft import power_method
 
