@@ -16,6 +16,7 @@
 
 from __future__ import annotations
 
+import inspect
 import math
 
 import numpy as np
@@ -66,14 +67,19 @@ def cg(operator, kspace_data, damp=0.0, x0=None, x_init=None, max_iter=10, tol=1
     ``toeplitz=True`` evaluates each gradient ``A^H D (A x - y)`` as
     ``G x - A^H D y`` with the operator's Toeplitz Gram form ``G`` (built
     once, ``gram_op``) and ``A^H D y`` computed once: the iterations then run
-    FFTs only, no spreading or interpolation.  Same recurrences otherwise.
+    FFTs only, no spreading or interpolation, and so do the power iterations
+    of the Lipschitz estimate where the operator offers that form.  Same
+    recurrences otherwise.
     """
     kind = kind_of(kspace_data)
     dev = _dev(operator)
     y = to_device(kspace_data, torch.complex64, dev)
     if x0 is not None:
         x0 = to_device(x0, torch.complex64, dev)
-    lipschitz_cst = float(operator.get_lipschitz_cst())
+    if toeplitz and "toeplitz" in inspect.signature(operator.get_lipschitz_cst).parameters:
+        lipschitz_cst = float(operator.get_lipschitz_cst(toeplitz=True))
+    else:
+        lipschitz_cst = float(operator.get_lipschitz_cst())
     old_density = None
     shape = operator.img_full_shape
     sq = operator.squeeze_dims
