@@ -335,6 +335,9 @@ def main():
                          "(N mixed 2D/3D CG requests) served vs plain per-call")
     ap.add_argument("--serving-rate", type=float, default=20.0,
                     help="Poisson arrival rate (requests/s) of the serving workload")
+    ap.add_argument("--serving-pass", type=int, default=12, metavar="N",
+                    help="requests of the bounded serving pass reported under the operator "
+                         "benchmark's 'serving' key (0: skip)")
     ap.add_argument("--stacked", action="store_true",
                     help="instead of the operator benchmark, time the stacked 2.5D operator on "
                          "a stack-of-stars 128^3, 8-coil workload against the full 3D operator "
@@ -508,6 +511,17 @@ def main():
                                                       / torch.linalg.vector_norm(rec))}
         log(f"cg toeplitz {cg['toeplitz']}")
 
+    tuned = None if op.tuning is None else dict(op.tuning)
+    serving = None
+    if args.serving_pass > 0:
+        from hipnufft.serving import benchmark as serving_benchmark
+
+        # the headline operator's workspaces are not needed any more
+        op = y = z = x = y_obs = rec = rec_t = plan = None
+        torch.cuda.empty_cache()
+        log(f"serving pass ({args.serving_pass} requests)")
+        serving = serving_benchmark(args.serving_pass, args.serving_rate, log=log)
+
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline (bounded sample)")
@@ -530,8 +544,7 @@ def main():
         "config": {"workload": "3D phyllotaxis radial 4096x512 (2,097,152 pts), 192^3, "
                                "16 coils SENSE, eps 1e-6, sigma 2",
                    "n_trans": T, "plan_setup_s": round(t_plan, 3), "plan_opts": plan_opts,
-                   "tuned": None if op.tuning is None else
-                   {k: v for k, v in op.tuning.items()}},
+                   "tuned": tuned},
         "cg_latency_ms": cg,
         "accuracy": acc,
         "stages": stage_summary,
@@ -546,6 +559,7 @@ def main():
                           "unit": "GB/s",
                           "frac": round(pbytes / (elapsed / args.steps) / 1e9 / PEAK_HBM_GBS, 4)},
         "cpu_baseline": cpu,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
 

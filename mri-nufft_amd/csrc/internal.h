@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <cstdlib>
+#include <cstring>
 #include <string>
 #include <vector>
 
@@ -17,6 +19,15 @@ constexpr int kMaxRadices = 16;
 
 // Record an error message for hipnufft_last_error().
 int set_error(int code, const std::string& msg);
+
+// Failure injection for tests of the error paths, driven from the host:
+// when the environment variable HIPNUFFT_INJECT_FAILURE equals ``site``, the
+// call fails at that site as an allocation failure would.  No kernel is
+// affected; unset, it costs one getenv per call.
+inline bool injected_failure(const char* site) {
+  const char* v = std::getenv("HIPNUFFT_INJECT_FAILURE");
+  return v && std::strcmp(v, site) == 0;
+}
 
 #define HN_HIP(call)                                                        \
   do {                                                                      \

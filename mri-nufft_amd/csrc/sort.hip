@@ -17,6 +17,41 @@
 namespace hn {
 namespace {
 
+// Stream-ordered temporaries, freed (hipFreeAsync on the owning stream) on
+// every exit of the scope that holds them, error returns included.
+class StreamTemps {
+ public:
+  explicit StreamTemps(hipStream_t s) : s_(s) {}
+  StreamTemps(const StreamTemps&) = delete;
+  StreamTemps& operator=(const StreamTemps&) = delete;
+  ~StreamTemps() { (void)release(); }
+
+  template <typename T>
+  hipError_t alloc(T** ptr, int64_t n) {
+    *ptr = nullptr;
+    const size_t bytes = sizeof(T) * (size_t)(n > 0 ? n : 1);
+    hipError_t e = hipMallocAsync((void**)ptr, bytes, s_);
+    if (e == hipSuccess) held_.push_back((void*)*ptr);
+    return e;
+  }
+
+  // frees everything held so far; returns the first error
+  hipError_t release() {
+    hipError_t first = hipSuccess;
+    for (void* q : held_) {
+      hipError_t e = hipFreeAsync(q, s_);
+      if (first == hipSuccess) first = e;
+    }
+    held_.clear();
+    return first;
+  }
+
+ private:
+  hipStream_t s_;
+  std::vector<void*> held_;
+};
+
+
 constexpr int kScanIPT = 8;                 // items per thread in scans
 constexpr int kScanChunk = kThreads * kScanIPT;
 constexpr int kSortIPT = 16;                // items per thread in radix passes
@@ -200,7 +235,8 @@ int radix_sort(uint32_t* k0, int32_t* v0, uint32_t* k1, int32_t* v1, int64_t n,
   if (n <= 1 || bits <= 0) return 0;
   const int nblk = (int)((n + kSortChunk - 1) / kSortChunk);
   int32_t* hist = nullptr;
-  HN_HIP(hipMallocAsync((void**)&hist, sizeof(int32_t) * 256 * (size_t)nblk, s));
+  StreamTemps tmp(s);
+  HN_HIP(tmp.alloc(&hist, 256 * (int64_t)nblk));
   for (int shift = 0; shift < bits; shift += 8) {
     hipLaunchKernelGGL(k_digit_hist, dim3(nblk), dim3(kThreads), 0, s, k0, n, shift, hist, nblk);
     HN_LAUNCH_CHECK();
@@ -212,7 +248,7 @@ int radix_sort(uint32_t* k0, int32_t* v0, uint32_t* k1, int32_t* v1, int64_t n,
     std::swap(k0, k1);
     std::swap(v0, v1);
   }
-  HN_HIP(hipFreeAsync(hist, s));
+  HN_HIP(tmp.release());
   // an odd number of passes leaves the result in the second buffer: copy back
   const int passes = (bits + 7) / 8;
   if (passes & 1) {
@@ -423,8 +459,9 @@ int scan_exclusive(const int32_t* in, int32_t* out, int64_t n, hipStream_t s, in
   int32_t last_in = 0;
   if (total)
     HN_HIP(hipMemcpyAsync(&last_in, in + n - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  StreamTemps tmp(s);
   int32_t* bsum = nullptr;
-  HN_HIP(hipMallocAsync((void**)&bsum, sizeof(int32_t) * (nblk + 1), s));
+  HN_HIP(tmp.alloc(&bsum, nblk + 1));
   hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nblk), dim3(kThreads), 0, s, in, out, bsum, n);
   HN_LAUNCH_CHECK();
   if (nblk > 1) {
@@ -442,7 +479,7 @@ int scan_exclusive(const int32_t* in, int32_t* out, int64_t n, hipStream_t s, in
     HN_HIP(hipStreamSynchronize(s));
     *total = (int64_t)last_in + last_out;
   }
-  HN_HIP(hipFreeAsync(bsum, s));
+  HN_HIP(tmp.release());
   return 0;
 }
 
@@ -461,15 +498,16 @@ static int per_set_offsets(const hipnufft_plan_s* p, const int32_t* scanned, int
   out.assign(K + 1, 0);
   out[K] = total;
   if (K == 1) return 0;
+  StreamTemps tmp(p->stream);
   int32_t* d = nullptr;
-  HN_HIP(hipMallocAsync((void**)&d, sizeof(int32_t) * K, p->stream));
+  HN_HIP(tmp.alloc(&d, K));
   hipLaunchKernelGGL(k_pick, dim3((unsigned)((K + 255) / 256)), dim3(256), 0, p->stream, scanned,
                      p->nbins, K, d);
   HN_LAUNCH_CHECK();
   std::vector<int32_t> h(K);
   HN_HIP(hipMemcpyAsync(h.data(), d, sizeof(int32_t) * K, hipMemcpyDeviceToHost, p->stream));
   HN_HIP(hipStreamSynchronize(p->stream));
-  HN_HIP(hipFree(d));
+  HN_HIP(tmp.release());
   for (int k = 0; k < K; ++k) out[k] = h[k];
   return 0;
 }
@@ -479,10 +517,11 @@ static int build_spread_items(hipnufft_plan_s* p) {
   const int64_t nkeys = p->nbins * p->nsets;
   const int S = p->spread_max;
   int rc;
+  StreamTemps tmp(s);
   int32_t *cnt = nullptr, *dcnt = nullptr, *dflag = nullptr;
-  HN_HIP(hipMallocAsync((void**)&cnt, sizeof(int32_t) * nkeys, s));
-  HN_HIP(hipMallocAsync((void**)&dcnt, sizeof(int32_t) * nkeys, s));
-  HN_HIP(hipMallocAsync((void**)&dflag, sizeof(int32_t) * nkeys, s));
+  HN_HIP(tmp.alloc(&cnt, nkeys));
+  HN_HIP(tmp.alloc(&dcnt, nkeys));
+  HN_HIP(tmp.alloc(&dflag, nkeys));
   const unsigned g = (unsigned)((nkeys + kThreads - 1) / kThreads);
   hipLaunchKernelGGL(k_item_count, dim3(g), dim3(kThreads), 0, s, p->list_start, nkeys, S, cnt,
                      dcnt, dflag);
@@ -513,6 +552,7 @@ static int build_spread_items(hipnufft_plan_s* p) {
   if (part_need > p->part_cap) {
     if (p->part) HN_HIP(hipFree(p->part));
     p->part = nullptr;
+    p->part_cap = 0;
     if (hipMalloc((void**)&p->part, sizeof(float2) * part_need) != hipSuccess)
       return set_error(HIPNUFFT_ERR_ALLOC, "cannot allocate the spreading scratch");
     p->part_cap = part_need;
@@ -524,9 +564,7 @@ static int build_spread_items(hipnufft_plan_s* p) {
   p->nitems = nitems;
   p->nslots = nslots;
   p->ndense = ndense;
-  HN_HIP(hipFreeAsync(cnt, s));
-  HN_HIP(hipFreeAsync(dcnt, s));
-  HN_HIP(hipFreeAsync(dflag, s));
+  HN_HIP(tmp.release());
   return 0;
 }
 
@@ -548,6 +586,13 @@ int build_points(hipnufft_plan_s* p, int nsets, const int64_t* Ms, const float* 
   for (int d = 0; d < D; ++d)
     if (!xd[d] && M > 0) return set_error(HIPNUFFT_ERR_ARG, "missing coordinate array");
 
+  // From here on the plan's point arrays are rebuilt: the plan is unusable
+  // (every transform returns HIPNUFFT_ERR_NOPTS) until the whole build has
+  // succeeded, so a failure part-way never leaves stale or half-written
+  // indices behind a usable plan.
+  p->has_pts = 0;
+  StreamTemps tmp(s);
+
   // persistent sorted arrays (one capacity for all of them)
   if (M > p->M_cap || !p->pt_idx) {
     for (int d = 0; d < 3; ++d) {
@@ -558,6 +603,7 @@ int build_points(hipnufft_plan_s* p, int nsets, const int64_t* Ms, const float* 
     }
     if (p->pt_idx) HN_HIP(hipFree(p->pt_idx));
     p->pt_idx = nullptr;
+    p->M_cap = 0;
     const int64_t cap = M > 0 ? M : 1;
     for (int d = 0; d < D; ++d) {
       HN_HIP(hipMalloc((void**)&p->pt_cell[d], sizeof(int32_t) * cap));
@@ -570,6 +616,7 @@ int build_points(hipnufft_plan_s* p, int nsets, const int64_t* Ms, const float* 
   if (nsets + 1 > p->set_off_cap || !p->set_off) {
     if (p->set_off) HN_HIP(hipFree(p->set_off));
     p->set_off = nullptr;
+    p->set_off_cap = 0;
     HN_HIP(hipMalloc((void**)&p->set_off, sizeof(int32_t) * (nsets + 1)));
     p->set_off_cap = nsets + 1;
   }
@@ -582,13 +629,13 @@ int build_points(hipnufft_plan_s* p, int nsets, const int64_t* Ms, const float* 
   p->nsets = nsets;
   p->set_off_h = soff;
   p->M = M;
-  p->has_pts = 1;
 
   const int64_t nkeys = p->nbins * nsets;
   if (nkeys + 1 > p->keys_cap || !p->bin_start) {
     if (p->bin_start) HN_HIP(hipFree(p->bin_start));
     if (p->list_start) HN_HIP(hipFree(p->list_start));
     p->bin_start = p->list_start = nullptr;
+    p->keys_cap = 0;
     HN_HIP(hipMalloc((void**)&p->bin_start, sizeof(int32_t) * (nkeys + 1)));
     HN_HIP(hipMalloc((void**)&p->list_start, sizeof(int32_t) * (nkeys + 1)));
     p->keys_cap = nkeys + 1;
@@ -603,6 +650,7 @@ int build_points(hipnufft_plan_s* p, int nsets, const int64_t* Ms, const float* 
     int rc0 = build_spread_items(p);  // empty tiles still write zeros
     if (rc0) return rc0;
     HN_HIP(hipStreamSynchronize(s));
+    p->has_pts = 1;
     return 0;
   }
 
@@ -611,13 +659,13 @@ int build_points(hipnufft_plan_s* p, int nsets, const int64_t* Ms, const float* 
   int32_t *va = nullptr, *vb = nullptr;
   int32_t* ctmp[3] = {nullptr, nullptr, nullptr};
   float* otmp[3] = {nullptr, nullptr, nullptr};
-  HN_HIP(hipMallocAsync((void**)&ka, sizeof(uint32_t) * M, s));
-  HN_HIP(hipMallocAsync((void**)&kb, sizeof(uint32_t) * M, s));
-  HN_HIP(hipMallocAsync((void**)&va, sizeof(int32_t) * M, s));
-  HN_HIP(hipMallocAsync((void**)&vb, sizeof(int32_t) * M, s));
+  HN_HIP(tmp.alloc(&ka, M));
+  HN_HIP(tmp.alloc(&kb, M));
+  HN_HIP(tmp.alloc(&va, M));
+  HN_HIP(tmp.alloc(&vb, M));
   for (int d = 0; d < D; ++d) {
-    HN_HIP(hipMallocAsync((void**)&ctmp[d], sizeof(int32_t) * M, s));
-    HN_HIP(hipMallocAsync((void**)&otmp[d], sizeof(float) * M, s));
+    HN_HIP(tmp.alloc(&ctmp[d], M));
+    HN_HIP(tmp.alloc(&otmp[d], M));
   }
 
   FoldArgs fa{};
@@ -670,11 +718,14 @@ int build_points(hipnufft_plan_s* p, int nsets, const int64_t* Ms, const float* 
     HN_LAUNCH_CHECK();
   }
 
+  if (injected_failure("setpts"))
+    return set_error(HIPNUFFT_ERR_ALLOC, "injected failure in setpts (HIPNUFFT_INJECT_FAILURE)");
+
   // interp work items: the points of interp_span bins along the last axis,
   // split into chunks of max_sub points
   {
     int32_t* cnt = nullptr;
-    HN_HIP(hipMallocAsync((void**)&cnt, sizeof(int32_t) * nkeys, s));
+    HN_HIP(tmp.alloc(&cnt, nkeys));
     const unsigned g = (unsigned)((nkeys + kThreads - 1) / kThreads);
     hipLaunchKernelGGL(k_sub_count, dim3(g), dim3(kThreads), 0, s, p->bin_start, nkeys,
                        (uint32_t)p->nbins, p->nt[2], p->interp_span, p->max_sub, cnt);
@@ -693,7 +744,6 @@ int build_points(hipnufft_plan_s* p, int nsets, const int64_t* Ms, const float* 
                        (uint32_t)p->nbins, p->nt[2], p->interp_span, p->max_sub, p->sub_tile,
                        p->sub_begin);
     HN_LAUNCH_CHECK();
-    HN_HIP(hipFreeAsync(cnt, s));
   }
 
   // spread lists: one entry per (point, touched tile), stably sorted by tile
@@ -720,9 +770,9 @@ int build_points(hipnufft_plan_s* p, int nsets, const int64_t* Ms, const float* 
     if (L > INT32_MAX / 2) return set_error(HIPNUFFT_ERR_ARG, "spread lists too large");
     uint32_t *lk0 = nullptr, *lk1 = nullptr;
     int32_t* lv1 = nullptr;
-    HN_HIP(hipMallocAsync((void**)&lk0, sizeof(uint32_t) * L, s));
-    HN_HIP(hipMallocAsync((void**)&lk1, sizeof(uint32_t) * L, s));
-    HN_HIP(hipMallocAsync((void**)&lv1, sizeof(int32_t) * L, s));
+    HN_HIP(tmp.alloc(&lk0, L));
+    HN_HIP(tmp.alloc(&lk1, L));
+    HN_HIP(tmp.alloc(&lv1, L));
     int64_t c = p->list_cap;
     if ((rc = grow(&p->list_pts, &c, L))) return rc;
     p->list_cap = c;
@@ -738,22 +788,13 @@ int build_points(hipnufft_plan_s* p, int nsets, const int64_t* Ms, const float* 
     const unsigned g = (unsigned)((L + 1 + kThreads - 1) / kThreads);
     hipLaunchKernelGGL(k_bin_start, dim3(g), dim3(kThreads), 0, s, lk0, L, nkeys, p->list_start);
     HN_LAUNCH_CHECK();
-    HN_HIP(hipFreeAsync(lk0, s));
-    HN_HIP(hipFreeAsync(lk1, s));
-    HN_HIP(hipFreeAsync(lv1, s));
   }
 
   if ((rc = build_spread_items(p))) return rc;
 
-  HN_HIP(hipFreeAsync(ka, s));
-  HN_HIP(hipFreeAsync(kb, s));
-  HN_HIP(hipFreeAsync(va, s));
-  HN_HIP(hipFreeAsync(vb, s));
-  for (int d = 0; d < D; ++d) {
-    HN_HIP(hipFreeAsync(ctmp[d], s));
-    HN_HIP(hipFreeAsync(otmp[d], s));
-  }
+  HN_HIP(tmp.release());
   HN_HIP(hipStreamSynchronize(s));
+  p->has_pts = 1;
   return 0;
 }
 

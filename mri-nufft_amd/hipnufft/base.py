@@ -1,6 +1,7 @@
 """Operator registry and base contract, mirroring mri-nufft's plugin interface.
 
-Restates ``src/mrinufft/operators/base.py``:
+The interface of ``src/mrinufft/operators/base.py`` (names, signatures,
+error types and messages), implemented for this backend:
 
 * ``check_backend`` / ``list_backends`` / ``get_operator``  (:45-179)
 * ``FourierOperatorBase`` registry via ``__init_subclass__`` (:182-218),
@@ -33,48 +34,57 @@ from ._array import kind_of
 DTYPE_R2C = {"float32": "complex64", "float64": "complex128"}
 
 
+def _registered(backend_name: str):
+    """(available, class) of a registered backend, or None."""
+    return FourierOperatorBase.interfaces.get(backend_name.lower())
+
+
 def check_backend(backend_name: str):
     """Availability flag of a registered backend (ValueError if unknown)."""
-    backend_name = backend_name.lower()
-    try:
-        return FourierOperatorBase.interfaces[backend_name][0]
-    except KeyError as e:
-        raise ValueError(f"unknown backend: '{backend_name}'") from e
+    entry = _registered(backend_name)
+    if entry is None:
+        raise ValueError(f"unknown backend: '{backend_name.lower()}'")
+    return entry[0]
 
 
 def list_backends(available_only=False):
-    """Registered backend names."""
-    return [
-        name
-        for name, (available, _) in FourierOperatorBase.interfaces.items()
-        if available or not available_only
-    ]
+    """Registered backend names (only the usable ones with ``available_only``)."""
+    names = list(FourierOperatorBase.interfaces)
+    return [n for n in names if check_backend(n)] if available_only else names
 
 
 def get_operator(backend_name: str, wrt_data: bool = False, wrt_traj: bool = False,
                  paired_batch: bool = False, *args, **kwargs):
-    """Return the operator class, an instance (when args are given) or an autograd wrapper."""
-    backend_name = backend_name.lower()
-    try:
-        available, operator = FourierOperatorBase.interfaces[backend_name]
-    except KeyError as exc:
-        raise ValueError(f"backend {backend_name} does not exist") from exc
+    """The operator class of ``backend_name``; an instance when constructor
+    arguments are given; wrapped for autograd when ``wrt_data`` /
+    ``wrt_traj`` is set (an instance is wrapped now, a class when it is
+    built).  ValueError for an unknown or unavailable backend."""
+    name = backend_name.lower()
+    entry = _registered(name)
+    if entry is None:
+        raise ValueError(f"backend {name} does not exist")
+    available, klass = entry
     if not available:
         raise ValueError(
-            f"backend {backend_name} found, but dependencies are not met."
-            f" ``pip install mri-nufft[{backend_name}]`` may solve the issue."
+            f"backend {name} found, but dependencies are not met."
+            f" ``pip install mri-nufft[{name}]`` may solve the issue."
         )
-    if args or kwargs:
-        operator = operator(*args, **kwargs)
-    if wrt_data or wrt_traj:
-        if isinstance(operator, partial):
-            raise ValueError("Cannot create autograd of a partial operator.")
-        if isinstance(operator, FourierOperatorBase):
-            operator = operator.make_autograd(wrt_data=wrt_data, wrt_traj=wrt_traj,
-                                              paired_batch=paired_batch)
-        else:
-            operator = partial(operator.with_autograd, wrt_data, wrt_traj, paired_batch)
-    return operator
+    target = klass(*args, **kwargs) if (args or kwargs) else klass
+    if not (wrt_data or wrt_traj):
+        return target
+    if isinstance(target, partial):
+        raise ValueError("Cannot create autograd of a partial operator.")
+    if isinstance(target, FourierOperatorBase):
+        return target.make_autograd(wrt_data=wrt_data, wrt_traj=wrt_traj,
+                                    paired_batch=paired_batch)
+    return partial(klass.with_autograd, wrt_data, wrt_traj, paired_batch)
+
+
+def _positive_count(value, what):
+    """``value`` as an int when it is a whole number >= 1, else ValueError."""
+    if value < 1 or int(value) != value:
+        raise ValueError(f"{what} should be a positive integer, {type(value)}")
+    return int(value)
 
 
 class FourierOperatorBase:
@@ -100,12 +110,13 @@ class FourierOperatorBase:
         self._grad_wrt_field_map = False
 
     def __init_subclass__(cls):
+        """Defining a subclass with a ``backend`` name registers it, with its
+        ``available`` flag (called once if it is a function)."""
         super().__init_subclass__()
-        available = getattr(cls, "available", True)
-        if callable(available):
-            available = available()
-        if backend := getattr(cls, "backend", None):
-            cls.interfaces[backend] = (available, cls)
+        name = getattr(cls, "backend", None)
+        if name:
+            flag = getattr(cls, "available", True)
+            cls.interfaces[name] = (flag() if callable(flag) else flag, cls)
 
     @property
     def log(self):
@@ -113,23 +124,17 @@ class FourierOperatorBase:
 
     # ------------------------------------------------------------------ shapes
     def check_shape(self, *, image=None, ksp=None):
-        """ValueError when the image / k-space trailing shape does not match."""
-        if image is not None:
-            image_shape = tuple(image.shape[-len(self.shape):])
-            if image_shape != self.shape:
-                raise ValueError(
-                    f"Image shape {image_shape} is not compatible "
-                    f"with the operator shape {self.shape}"
-                )
-        if ksp is not None:
-            kspace_shape = ksp.shape[-1]
-            if kspace_shape != self.n_samples:
-                raise ValueError(
-                    f"Kspace samples {kspace_shape} is not compatible "
-                    f"with the operator samples {self.n_samples}"
-                )
+        """ValueError when the image's trailing axes differ from the operator
+        shape or the k-space's last axis from the number of samples."""
         if image is None and ksp is None:
             raise ValueError("Nothing to check, provides image or ksp arguments")
+        got = None if image is None else tuple(image.shape[-self.ndim:])
+        if got is not None and got != self.shape:
+            raise ValueError(f"Image shape {got} is not compatible "
+                             f"with the operator shape {self.shape}")
+        if ksp is not None and ksp.shape[-1] != self.n_samples:
+            raise ValueError(f"Kspace samples {ksp.shape[-1]} is not compatible "
+                             f"with the operator samples {self.n_samples}")
 
     def _safe_squeeze(self, arr):
         """Squeeze the coil axis, then the batch axis, when they are singleton."""
@@ -165,9 +170,7 @@ class FourierOperatorBase:
 
     @n_coils.setter
     def n_coils(self, n_coils):
-        if n_coils < 1 or not int(n_coils) == n_coils:
-            raise ValueError(f"n_coils should be a positive integer, {type(n_coils)}")
-        self._n_coils = int(n_coils)
+        self._n_coils = _positive_count(n_coils, "n_coils")
 
     @property
     def n_batchs(self):
@@ -175,9 +178,7 @@ class FourierOperatorBase:
 
     @n_batchs.setter
     def n_batchs(self, n_batchs):
-        if n_batchs < 1 or not int(n_batchs) == n_batchs:
-            raise ValueError(f"n_batchs should be a positive integer, {type(n_batchs)}")
-        self._n_batchs = int(n_batchs)
+        self._n_batchs = _positive_count(n_batchs, "n_batchs")
 
     @property
     def img_full_shape(self):
@@ -248,34 +249,39 @@ class FourierOperatorBase:
             raise NotImplementedError("Toeplitz Gram operator is not implemented by hipnufft")
         return self.adj_op(self.op(data))
 
-    def compute_density(self, method=None):
-        """Set the density compensation (array, True/'pipe', dict, name or callable)."""
+    def _density_estimator(self, method):
+        """(function, keyword arguments) of a density request: True or
+        "pipe" (Pipe on this backend), a registered name, a dict with a
+        "name" and that method's arguments, or a callable."""
         from .density import get_density
 
-        kwargs = {}
-        if method is None or method is False:
-            self._density = None
-            return
-        if isinstance(method, (np.ndarray, torch.Tensor)):
-            self.density = method
-            return
         if method is True or method == "pipe":
-            method_ = get_density("pipe")
-            kwargs["backend"] = self.backend
-        elif isinstance(method, dict) and isinstance(method.get("name"), str):
-            kwargs = {k: v for k, v in method.items() if k != "name"}
-            if method["name"] == "pipe" and "backend" not in kwargs:
-                kwargs["backend"] = self.backend
-            method_ = get_density(method["name"])
-        elif isinstance(method, str):
-            method_ = get_density(method)
-        elif callable(method):
-            method_ = method
+            return get_density("pipe"), {"backend": self.backend}
+        if isinstance(method, dict) and isinstance(method.get("name"), str):
+            opts = {k: v for k, v in method.items() if k != "name"}
+            if method["name"] == "pipe":
+                opts.setdefault("backend", self.backend)
+            return get_density(method["name"]), opts
+        if isinstance(method, str):
+            return get_density(method), {}
+        if callable(method):
+            return method, {}
+        raise ValueError(f"Unknown density method: {method}")
+
+    def compute_density(self, method=None):
+        """Set the density compensation from an array, or from an estimator
+        (see ``_density_estimator``) run on the samples; None / False
+        removes it.  The first estimator used is kept to re-run on
+        ``update_samples``."""
+        if method is None or method is False:
+            self.density = None
+        elif isinstance(method, (np.ndarray, torch.Tensor)):
+            self.density = method
         else:
-            raise ValueError(f"Unknown density method: {method}")
-        self.density = method_(self.samples, self.shape, **kwargs)
-        if self._density_method is None:
-            self._density_method = partial(method_, **kwargs)
+            estimate, opts = self._density_estimator(method)
+            self.density = estimate(self.samples, self.shape, **opts)
+            if self._density_method is None:
+                self._density_method = partial(estimate, **opts)
 
     def compute_smaps(self, method=None):
         """Set the sensitivity maps (array or callable(samples, shape, ...))."""
@@ -371,52 +377,64 @@ def random_start(shape, device, seed=None):
     return torch.complex(re, im)
 
 
+def _l2(v):
+    if isinstance(v, torch.Tensor):
+        return torch.linalg.vector_norm(v.reshape(-1))
+    return np.linalg.norm(v)
+
+
 def power_method(max_iter, operator, norm_func=None, x=None):
-    """Largest eigenvalue of A^H A (``base.py:1155-1221``).
+    """Largest eigenvalue of A^H A by power iterations (interface and stopping
+    rule of ``operators/base.py:1155-1221``).
 
-    Returns (norm, eigenvector).  Works on numpy arrays or torch tensors.
+    ``operator`` is an operator (A^H A = adj_op(op(.))) or a callable applying
+    A^H A.  Starting from ``x`` (required for a callable; otherwise
+    ``U[0, 1)`` on the host, complex), each step applies A^H A to the unit
+    vector and takes the norm of the result as the estimate.  The iterations
+    stop when two successive norms differ by less than 1e-6 (the first one
+    against the start vector's norm), after at most ``max_iter`` steps; a
+    warning is issued when all ``max_iter`` steps ran.  Returns
+    (estimate, unit eigenvector estimate); the estimate is a python float
+    when the start vector was drawn here.  Works on numpy arrays and torch
+    tensors.
     """
-
-    def AHA(v):
-        if callable(operator) and not isinstance(operator, FourierOperatorBase):
-            return operator(v)
-        return operator.adj_op(operator.op(v))
-
-    if norm_func is None:
-        def norm_func(v):
-            if isinstance(v, torch.Tensor):
-                return torch.linalg.vector_norm(v.reshape(-1))
-            return np.linalg.norm(v)
-
-    return_as_is = True
-    if x is None and isinstance(operator, FourierOperatorBase):
-        return_as_is = False
+    if callable(operator) and not isinstance(operator, FourierOperatorBase):
+        gram = operator
+    else:
+        def gram(v):
+            return operator.adj_op(operator.op(v))
+    norm = norm_func or _l2
+    drawn = x is None
+    if drawn:
+        if not isinstance(operator, FourierOperatorBase):
+            raise ValueError("If operator is not a FourierOperatorBase, an initial x should "
+                             "be provided.")
         x = np.random.random(operator.shape).astype(operator.cpx_dtype)
-    elif x is None:
-        raise ValueError("If operator is not a FourierOperatorBase, an initial x should be provided.")
-    x_norm = norm_func(x)
-    x = x / x_norm
-    i = 0
-    for i in range(max_iter):  # noqa: B007
-        x_new = AHA(x)
-        x_new_norm = norm_func(x_new)
-        x_new = x_new / x_new_norm
-        if abs(float(x_norm) - float(x_new_norm)) < 1e-6:
+    last = norm(x)
+    vec = x / last
+    estimate = last
+    steps = 0
+    while steps < max_iter:
+        steps += 1
+        image = gram(vec)
+        estimate = norm(image)
+        vec = image / estimate
+        if abs(float(last) - float(estimate)) < 1e-6:
             break
-        x_norm = x_new_norm
-        x = x_new
-    if i == max_iter - 1:
+        last = estimate
+    if max_iter and steps == max_iter:
         warnings.warn("Lipschitz constant did not converge")
-    if return_as_is:
-        return x_new_norm, x_new
-    return float(x_new_norm), x_new
+    return (float(estimate) if drawn else estimate), vec
 
 
 class _ToggleGradPlanMixin:
-    """Swap in the trajectory-gradient plan (``base.py:1224-1238``)."""
+    """Swaps the operator's type-2 plan with its +1-sign plan (and its coil
+    maps with their conjugates) for the trajectory-gradient products
+    (``base.py:1224-1238``)."""
 
     @contextmanager
     def grad_traj_plan(self):
+        """Run the body on the gradient plan; the swap is undone on exit."""
         self.toggle_grad_traj()
         try:
             yield

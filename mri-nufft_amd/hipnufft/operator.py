@@ -217,12 +217,19 @@ class MRIHipNUFFT(FourierOperatorBase, _ToggleGradPlanMixin):
     def update_samples(self, new_samples, *, unsafe=False):
         """Re-bin the samples on the device and recompute the density (``cufinufft.py:317-347``)."""
         with torch.cuda.device(self.device):
-            if not unsafe:
-                traj = proper_trajectory(new_samples, normalize="pi")
-                self._samples = to_device(traj, torch.float32, self.device)
+            if unsafe:
+                traj = new_samples
             else:
-                self._samples = new_samples
-            self.raw_op.set_pts(self._samples)
+                traj = to_device(proper_trajectory(new_samples, normalize="pi"), torch.float32,
+                                 self.device)
+            if traj.ndim != 2 or traj.shape[1] != self.ndim:
+                raise ValueError(f"samples should have shape (M, {self.ndim}), "
+                                 f"got {tuple(traj.shape)}")
+            # the new samples are committed only once the device sort has
+            # succeeded; a failed sort leaves the plan without points, so
+            # every transform raises until the next successful update
+            self.raw_op.set_pts(traj)
+            self._samples = traj
             self._toeplitz_kernel = None
             if self._density_method is not None:
                 self.compute_density(self._density_method)

@@ -36,7 +36,7 @@ import numpy as np
 import torch
 
 from . import _capi
-from ._array import proper_trajectory
+from ._array import proper_trajectory, to_device
 from .operator import MRIHipNUFFT
 
 
@@ -57,15 +57,44 @@ class ReconRequest:
 
 @dataclass
 class ReconResult:
+    """A finished request: its image, or ``image=None`` and the reason in
+    ``error`` when the request was rejected or failed."""
+
     rid: int
     image: np.ndarray
     arrival: float
     admitted: float
     finished: float
+    error: str = None
+
+    @property
+    def ok(self):
+        return self.error is None
 
     @property
     def latency(self):
         return self.finished - self.arrival
+
+
+def validate_request(req: ReconRequest):
+    """Raise ValueError unless the request's arrays agree with each other:
+    samples (M, d) with d = len(shape) in 2..3, k-space (C, M), coil maps
+    (C, *shape), a non-negative iteration budget."""
+    shape = tuple(int(n) for n in req.shape)
+    if len(shape) not in (2, 3) or min(shape) < 1:
+        raise ValueError(f"shape {req.shape} is not a 2D or 3D image shape")
+    samples = np.shape(req.samples)
+    if len(samples) < 2 or samples[-1] != len(shape):
+        raise ValueError(f"samples of shape {samples} do not have {len(shape)} coordinates")
+    M = int(np.prod(samples[:-1]))
+    ksp = np.shape(req.kspace)
+    if len(ksp) != 2 or ksp[1] != M or ksp[0] < 1:
+        raise ValueError(f"k-space of shape {ksp} is not (coils, {M})")
+    maps = np.shape(req.smaps)
+    if tuple(maps) != (ksp[0], *shape):
+        raise ValueError(f"coil maps of shape {maps} are not {(ksp[0], *shape)}")
+    if int(req.max_iter) < 0:
+        raise ValueError("max_iter must be >= 0")
 
 
 def estimate_bytes(shape, n_samples, n_coils, upsampfac=2.0, toeplitz=False):
@@ -127,12 +156,16 @@ class _Plan:
         self.in_use = False
 
     def bind(self, req, stream):
-        """Point the resident operator at one request's coils and maps."""
+        """Point the resident operator at one request's coils and maps (cast
+        to complex64; their shape is checked against (C, *shape))."""
         C = int(req.kspace.shape[0])
         op = self.op
+        if tuple(np.shape(req.smaps)) != (C, *op.shape):
+            raise ValueError(f"coil maps of shape {np.shape(req.smaps)} are not "
+                             f"{(C, *op.shape)}")
         with torch.cuda.stream(stream):
             op.n_coils = C
-            op._smaps = torch.from_numpy(np.ascontiguousarray(req.smaps)).to(op.device)
+            op._smaps = to_device(req.smaps, torch.complex64, op.device).reshape(C, *op.shape)
 
 
 class _CGJob:
@@ -261,7 +294,7 @@ class ReconServer:
         self.results = {}
         self._realtime = True
         self._next_id = 0
-        self.stats = dict(admitted=0, plan_hits=0, plan_builds=0, evictions=0)
+        self.stats = dict(admitted=0, plan_hits=0, plan_builds=0, evictions=0, failed=0)
 
     # ------------------------------------------------------------- memory
     def _resident_bytes(self):
@@ -278,12 +311,33 @@ class ReconServer:
         return True
 
     # ----------------------------------------------------------- requests
-    def submit(self, req: ReconRequest):
+    def submit(self, req: ReconRequest, now=0.0):
+        """Queue a request.  One that is malformed or can never fit the HBM
+        budget is answered at once with a failed result; the others are
+        unaffected."""
         if req.rid < 0:
             req.rid = self._next_id
         self._next_id = max(self._next_id, req.rid) + 1
+        try:
+            validate_request(req)
+            C = int(req.kspace.shape[0])
+            need = estimate_bytes(req.shape, req.kspace.shape[1],
+                                  max(C, plan_ntrans(req.shape, C)),
+                                  toeplitz=self.toeplitz and toeplitz_supported(req.shape))
+            if need > self.budget:
+                raise ValueError(f"request {req.rid} needs {need} bytes, over the HBM budget "
+                                 f"of {self.budget}")
+        except ValueError as exc:
+            self._fail(req, now, now, exc)
+            return req.rid
         self.waiting.append(req)
         return req.rid
+
+    def _fail(self, req, admitted, now, exc):
+        arr = req.arrival if self._realtime else 0.0
+        self.results[req.rid] = ReconResult(req.rid, None, arr, admitted, now,
+                                            error=f"{type(exc).__name__}: {exc}")
+        self.stats["failed"] += 1
 
     def _key(self, req):
         if req.traj_key is not None:
@@ -306,42 +360,75 @@ class ReconServer:
             need = estimate_bytes(req.shape, req.kspace.shape[1],
                                   max(C, plan_ntrans(req.shape, C)),
                                   toeplitz=self.toeplitz and toeplitz_supported(req.shape))
-            if need > self.budget:
-                raise ValueError(f"request {req.rid} needs {need} bytes, over the HBM budget")
             if not self._make_room(need):
                 return False
         si = self.free_streams.pop()
         stream = self.streams[si]
-        if plan is None:
-            traj = proper_trajectory(req.samples, normalize="pi")
-            plan = _Plan(key, traj, tuple(req.shape), C, stream, self.seed, self.toeplitz)
-            self.plans[key] = plan
-            self.stats["plan_builds"] += 1
-        else:
-            self.plans.move_to_end(key)
-            self.stats["plan_hits"] += 1
-        plan.in_use = True
-        plan.bind(req, stream)
-        dev = plan.op.device
-        with torch.cuda.stream(stream):
-            y = torch.from_numpy(np.ascontiguousarray(req.kspace, dtype=np.complex64)).to(dev)
-            y = y.reshape(1, C, -1)
-            job = _CGJob(plan, y, req.max_iter, req.tol, stream)
+        built = plan is None
+        try:
+            if built:
+                traj = proper_trajectory(req.samples, normalize="pi")
+                plan = _Plan(key, traj, tuple(req.shape), C, stream, self.seed, self.toeplitz)
+                self.plans[key] = plan
+                self.stats["plan_builds"] += 1
+            else:
+                self.plans.move_to_end(key)
+                self.stats["plan_hits"] += 1
+            plan.in_use = True
+            plan.bind(req, stream)
+            dev = plan.op.device
+            with torch.cuda.stream(stream):
+                y = to_device(req.kspace, torch.complex64, dev).reshape(1, C, -1)
+                job = _CGJob(plan, y, req.max_iter, req.tol, stream)
+        except Exception as exc:  # this request fails; the server and other requests go on
+            self._release(key, plan, si, drop_plan=built)
+            self._fail(req, now, now, exc)
+            return True
         self.active.append((req, job, si, now))
         self.stats["admitted"] += 1
         return True
 
+    def _release(self, key, plan, si, drop_plan=False):
+        """Give back a slot's stream; the plan becomes idle, or is dropped
+        when it may hold a half-built or failed state."""
+        self.free_streams.append(si)
+        if plan is not None:
+            plan.in_use = False
+        if drop_plan and self.plans.get(key) is plan:
+            del self.plans[key]
+
     def _retire(self, now):
         keep = []
         for req, job, si, adm in self.active:
-            if job.finished():
-                img = job.result()
-                job.plan.in_use = False
-                self.free_streams.append(si)
+            try:
+                done = job.finished()
+                img = job.result() if done else None
+            except Exception as exc:
+                self._release(job.plan.key, job.plan, si, drop_plan=True)
+                self._fail(req, adm, now, exc)
+                continue
+            if done:
+                self._release(job.plan.key, job.plan, si)
                 arr = req.arrival if self._realtime else 0.0
                 self.results[req.rid] = ReconResult(req.rid, img, arr, adm, now)
             else:
                 keep.append((req, job, si, adm))
+        self.active = keep
+
+    def _advance_all(self, now):
+        """Enqueue the next iterations of every active request; a request
+        whose work fails is answered with its error and its plan dropped."""
+        keep = []
+        for entry in self.active:
+            req, job, si, adm = entry
+            try:
+                for _ in range(self.depth if self.depth > 0 else job.max_iter):
+                    job.advance(self.use_graph)
+            except Exception as exc:
+                self._release(job.plan.key, job.plan, si, drop_plan=True)
+                self._fail(req, adm, now, exc)
+                continue
+            keep.append(entry)
         self.active = keep
 
     def run(self, requests, realtime=True):
@@ -358,7 +445,7 @@ class ReconServer:
         while pending or self.waiting or self.active:
             now = time.perf_counter() - t0
             while pending and (not realtime or pending[0].arrival <= now):
-                self.submit(pending.popleft())
+                self.submit(pending.popleft(), now)
             if self.waiting and self.free_streams:
                 # oldest first; a request whose plan is busy is passed over
                 keep = deque()
@@ -367,9 +454,7 @@ class ReconServer:
                     if not (self.free_streams and self._try_admit(r, now)):
                         keep.append(r)
                 self.waiting = keep
-            for _, job, _, _ in self.active:
-                for _ in range(self.depth if self.depth > 0 else job.max_iter):
-                    job.advance(self.use_graph)
+            self._advance_all(now)
             self._retire(time.perf_counter() - t0)
             if not self.active and not self.waiting and pending and realtime:
                 time.sleep(max(0.0, min(pending[0].arrival - (time.perf_counter() - t0), 0.05)))
@@ -390,13 +475,19 @@ def serve_plain(requests, realtime=True, graph=False):
             if wait > 0:
                 time.sleep(wait)
         adm = time.perf_counter() - t0
-        op = MRIHipNUFFT(req.samples, tuple(req.shape), n_coils=req.kspace.shape[0],
-                         smaps=req.smaps, squeeze_dims=False)
-        y = torch.from_numpy(np.ascontiguousarray(req.kspace, dtype=np.complex64)).to(op.device)
-        img = op.pinv_solver(y.reshape(1, req.kspace.shape[0], -1), optim="cg",
-                             max_iter=req.max_iter, tol=req.tol, graph=graph)
-        img = img.reshape(req.shape).cpu().numpy()
         arr = req.arrival if realtime else 0.0
+        try:
+            validate_request(req)
+            op = MRIHipNUFFT(req.samples, tuple(req.shape), n_coils=req.kspace.shape[0],
+                             smaps=req.smaps, squeeze_dims=False)
+            y = to_device(req.kspace, torch.complex64, op.device)
+            img = op.pinv_solver(y.reshape(1, req.kspace.shape[0], -1), optim="cg",
+                                 max_iter=req.max_iter, tol=req.tol, graph=graph)
+            img = img.reshape(req.shape).cpu().numpy()
+        except Exception as exc:
+            out[req.rid] = ReconResult(req.rid, None, arr, adm, time.perf_counter() - t0,
+                                       error=f"{type(exc).__name__}: {exc}")
+            continue
         out[req.rid] = ReconResult(req.rid, img, arr, adm, time.perf_counter() - t0)
     return out
 
@@ -448,7 +539,7 @@ def synth_kspace(traj, shape, C, smaps, img, op=None):
     an operator of the same trajectory to reuse (its maps are replaced)."""
     if op is None:
         op = MRIHipNUFFT(traj, shape, n_coils=C, smaps=None, squeeze_dims=False)
-    op._smaps = torch.from_numpy(np.ascontiguousarray(smaps)).to(op.device)
+    op._smaps = to_device(smaps, torch.complex64, op.device).reshape(C, *shape)
     y = op.op(torch.from_numpy(img).to(op.device)).reshape(C, -1)
     return y.cpu().numpy(), op
 
@@ -472,9 +563,14 @@ def workload(n_requests, rate_hz, seed=0, kinds=KINDS, max_iter=10):
 
 
 def summarize(results, requests=None):
-    lat = np.array([r.latency for r in results.values()])
-    span = max(r.finished for r in results.values()) - min(r.arrival for r in results.values())
-    return dict(n=len(lat), p50_ms=float(np.percentile(lat, 50) * 1e3),
+    """Latency percentiles and throughput over the successful results."""
+    good = [r for r in results.values() if r.ok]
+    failed = len(results) - len(good)
+    if not good:
+        return dict(n=0, failed=failed)
+    lat = np.array([r.latency for r in good])
+    span = max(r.finished for r in good) - min(r.arrival for r in good)
+    return dict(n=len(lat), failed=failed, p50_ms=float(np.percentile(lat, 50) * 1e3),
                 p99_ms=float(np.percentile(lat, 99) * 1e3),
                 requests_per_s=float(len(lat) / span) if span > 0 else math.inf)
 

@@ -22,12 +22,20 @@ mri         the MRI operator layer (SENSE expand/combine, density, batching,
             normalisation, data consistency) restated from
             ``operators/base.py:949-1139`` and
             ``operators/interfaces/cufinufft.py:478-1178``.
-solvers     power method, CG, LSQR, LSMR and Pipe density, restated from
-            ``operators/base.py:1155-1221``, ``extras/optim.py:213-902`` and
-            ``operators/interfaces/cufinufft.py:1281-1331``.
-trajectories  the benchmark-input generators restated from
-            ``trajectories/inits/radial.py``, ``inits/spiral.py``,
-            ``inits/cones.py`` and ``tools/transforms.py``.
+solvers     power method, CG and Pipe density, restated from
+            ``operators/base.py:1155-1221``, ``extras/optim.py:801-902`` and
+            ``operators/interfaces/cufinufft.py:1281-1331``; LSMR as
+            scipy's published implementation, which the reference's
+            ``extras/optim.py:498-798`` adapts; the exact SENSE operator
+            as a scipy LinearOperator.
+smaps       low-frequency coil-map estimation (window, blur, root sum of
+            squares, LSQR as scipy's published implementation) restated
+            from ``extras/smaps.py:83-306``.
+stacked     the stacked 2.5D operator (z DFT and plane selection) restated
+            from ``operators/stacked.py:447-790``.
+orc         the off-resonance operator and its interpolator factorizations
+            restated from ``operators/off_resonance.py:230-302`` and
+            ``extras/field_map.py:214-566``.
 
 How each function is pinned (checked independently of its own formula) is
 listed in the docstring of each module; the tests live in

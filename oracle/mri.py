@@ -10,8 +10,10 @@ and the cufinufft device paths (``operators/interfaces/cufinufft.py:478-1178``):
 * ``norm = sqrt(prod(N) 2^d)`` (``base.py:837-840``).
 
 The raw transform is either the exact FP64 NDFT (``exact=True``) or the FP64
-ES-NUFFT of ``oracle.nufft``.  Pinned by ``tests/test_oracle_mri.py``
-(NDFT vs NUFFT agreement, adjointness, DC == adj(op(x) - y)).
+ES-NUFFT of ``oracle.nufft``.  Pinned by ``tests/test_oracle_solvers.py``
+(``test_nufft_operator_matches_exact``: NDFT vs NUFFT agreement and
+adjointness; ``test_dc_is_adj_of_residual_with_density_once``: DC ==
+adj(op(x) - y) with the density applied once, against the explicit form).
 """
 
 from __future__ import annotations

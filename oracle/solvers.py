@@ -8,20 +8,32 @@
   complex ``max(0, beta)``, stop at ``||g|| <= tol``);
 * ``pipe``         <- ``operators/interfaces/cufinufft.py:1281-1331`` (fixed
   point ``d <- d / |A_s A_s^H d|`` with spread/interp-only operators on the
-  image grid, then ``d /= mean|A^H(d A 1)|``).
+  image grid, then ``d /= mean|A^H(d A 1)|``);
+* ``lsmr``         <- ``scipy.sparse.linalg.lsmr`` (Fong & Saunders), the
+  published implementation the reference's ``lsmr`` (``extras/optim.py:498-798``)
+  is adapted from, called directly, with the reference's defaults;
+* ``sense_linop``  the exact FP64 SENSE operator (``oracle/ndft.py`` matrix,
+  normalised, density-free) of one batch as a scipy ``LinearOperator``, for
+  the scipy solvers.
 
 Pinned by ``tests/test_oracle_solvers.py``: power method against the dense
-spectral norm; CG against scipy's least-squares solution on a tiny exact
-problem (monotone residual, convergence) and against a hand-unrolled first
-step; Pipe against its fixed-point property on a Cartesian grid, where the
-exact density is uniform.
+spectral norm; CG against the least-squares solution on a tiny exact problem
+(monotone residual, convergence) and against a hand-unrolled first step;
+Pipe on a Cartesian grid (uniform), on radial trajectories by its fixed
+point (|A_s A_s^H d| -> 1 with independently built spread / interp-only
+operators), by the normalised PSF energy (mean|A^H(d A 1)| = 1 with the
+exact NDFT operator) and by the reference's correlation bars against |k|;
+LSMR converging to the (damped) least-squares solution; ``sense_linop``
+against the dense SENSE matrix.
 """
 
 from __future__ import annotations
 
 import numpy as np
+from scipy.sparse.linalg import LinearOperator
+from scipy.sparse.linalg import lsmr as _lsmr
 
-from . import nufft
+from . import ndft, nufft
 from .mri import Operator
 
 
@@ -87,3 +99,28 @@ def pipe(samples, shape, max_iter=10, osf=2.0, normalize=True, eps=1e-6):
         rec = op.adj_op(d * op.op(np.ones(shape)).ravel())
         d = d / np.mean(np.abs(rec))
     return np.abs(d)
+
+
+def lsmr(A, b, max_iter, damp=0.0, atol=1e-6, btol=1e-6, conlim=1e8):
+    """LSMR iterates of ``min ||A x - b||^2 + damp^2 ||x||^2`` from x = 0
+    (the reference's defaults: atol = btol = 1e-6, conlim 1e8)."""
+    return _lsmr(A, b, damp=damp, atol=atol, btol=btol, conlim=conlim, maxiter=max_iter)[0]
+
+
+def sense_linop(samples, shape, smaps=None, n_coils=1):
+    """Exact normalised SENSE (or calibrationless, ``smaps=None``) operator of
+    one batch on flattened vectors: image (N,) or (C N,) -> k-space (C M,)."""
+    F = ndft.fourier_matrix(samples, shape, normalize=True)
+    M, N = F.shape
+    if smaps is None:
+        C = n_coils
+        return LinearOperator(
+            (C * M, C * N), dtype=np.complex128,
+            matvec=lambda x: (F @ x.reshape(C, N).T).T.reshape(-1),
+            rmatvec=lambda y: (F.conj().T @ y.reshape(C, M).T).T.reshape(-1))
+    S = np.asarray(smaps, dtype=np.complex128).reshape(-1, N)
+    C = S.shape[0]
+    return LinearOperator(
+        (C * M, N), dtype=np.complex128,
+        matvec=lambda x: (F @ (S * x.reshape(1, N)).T).T.reshape(-1),
+        rmatvec=lambda y: np.sum(np.conj(S) * (F.conj().T @ y.reshape(C, M).T).T, 0))

@@ -100,3 +100,96 @@ def test_requests_overlap_on_device():
     first.max_iter = 30
     res = ReconServer(max_slots=2).run([first, second], realtime=True)
     assert res[second.rid].admitted < res[first.rid].finished
+
+
+def test_invalid_requests_do_not_disturb_others():
+    """Malformed requests (coil maps for the wrong coil count, k-space of the
+    wrong length, over the HBM budget) placed between good ones are answered
+    with an error; the good results are bit-identical to their alone runs,
+    and the server stays usable."""
+    from hipnufft.serving import ReconRequest, ReconServer
+
+    good = _requests()[:3]
+    bad_maps = ReconRequest(good[0].samples, good[0].shape, good[0].kspace,
+                            good[0].smaps[:2], max_iter=6, traj_key="radA", rid=10)
+    bad_ksp = ReconRequest(good[2].samples, good[2].shape, good[2].kspace[:, :-5],
+                           good[2].smaps, max_iter=6, rid=11)
+    huge_shape = (2048, 2048, 64)
+    huge = ReconRequest(rand_traj(100, 3, 5), huge_shape, rand_cplx((1, 100), 6),
+                        np.ones((1, 1, 1, 1), np.complex64), max_iter=3, rid=12)
+    huge.smaps = np.broadcast_to(np.ones((1,) + huge_shape, np.complex64), (1,) + huge_shape)
+    mix = [good[0], bad_maps, good[1], bad_ksp, huge, good[2]]
+    srv = ReconServer(hbm_budget_bytes=8 << 30, max_slots=4)
+    res = srv.run(mix, realtime=False)
+    assert sorted(res) == sorted(r.rid for r in mix)
+    for r in (bad_maps, bad_ksp, huge):
+        assert not res[r.rid].ok and res[r.rid].image is None
+    assert "coil maps" in res[bad_maps.rid].error
+    assert "k-space" in res[bad_ksp.rid].error
+    assert "budget" in res[huge.rid].error
+    assert srv.stats["failed"] == 3
+    for r in good:
+        alone = ReconServer(max_slots=1).run([r], realtime=False)[r.rid]
+        assert res[r.rid].ok
+        np.testing.assert_array_equal(res[r.rid].image, alone.image)
+    # the same server serves again, with every plan idle
+    assert all(not p.in_use for p in srv.plans.values())
+    again = srv.run([good[1]], realtime=False)
+    np.testing.assert_array_equal(again[good[1].rid].image, res[good[1].rid].image)
+
+
+def test_complex128_maps_are_cast():
+    """Coil maps given as complex128 give the complex64 maps' result, in the
+    server and in the plain path."""
+    import copy
+
+    from hipnufft.serving import ReconServer, serve_plain
+
+    r64 = _requests()[3]
+    r128 = copy.copy(r64)
+    r128.smaps = r64.smaps.astype(np.complex128)
+    a = ReconServer(max_slots=1).run([r64], realtime=False)[r64.rid].image
+    b = ReconServer(max_slots=1).run([r128], realtime=False)[r128.rid].image
+    np.testing.assert_array_equal(a, b)
+    p = serve_plain([r128], realtime=False)[r128.rid]
+    assert p.ok and rel_l2(p.image, a) <= 1e-4
+
+
+def _serving_kinds():
+    """One request of each kind of the serving workload at its real size:
+    2D radial 256^2 (256 x 512 samples, 32 coils), 2D spiral 384^2
+    (96 x 4096, 16 coils), 3D stack of stars 128^3 (256 x 256 x 128
+    partitions, 8 coils); 10 CG iterations each."""
+    from hipnufft.serving import ReconRequest, synth_kspace
+    from hipnufft.trajectories import (gaussian_smaps, initialize_2D_radial,
+                                       initialize_2D_spiral, stack)
+
+    rng = np.random.default_rng(55)
+    specs = [("radial2d", initialize_2D_radial(256, 512).reshape(-1, 2), (256, 256), 32),
+             ("spiral2d", initialize_2D_spiral(96, 4096).reshape(-1, 2), (384, 384), 16),
+             ("stars3d", stack(initialize_2D_radial(256, 256), 128).reshape(-1, 3),
+              (128, 128, 128), 8)]
+    reqs = []
+    for i, (key, traj, shape, C) in enumerate(specs):
+        traj = traj.astype(np.float32)
+        smaps = gaussian_smaps(C, shape, seed=int(rng.integers(1 << 30)))
+        img = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64)
+        y, _ = synth_kspace(traj, shape, C, smaps, img)
+        reqs.append(ReconRequest(traj, shape, y, smaps, max_iter=10, traj_key=(key,),
+                                 arrival=0.0, rid=i))
+    return reqs
+
+
+def test_serving_kinds_alone_mix_and_plain():
+    from hipnufft.serving import ReconServer, serve_plain
+
+    reqs = _serving_kinds()
+    mix = ReconServer(max_slots=8).run(reqs, realtime=False)
+    plain = serve_plain(reqs, realtime=False)
+    for r in reqs:
+        alone = ReconServer(max_slots=1).run([r], realtime=False)[r.rid]
+        e_mix = rel_l2(mix[r.rid].image, alone.image)
+        e_plain = rel_l2(mix[r.rid].image, plain[r.rid].image)
+        print(f"{r.traj_key[0]}: mix vs alone {e_mix:.1e}, served vs plain {e_plain:.1e}")
+        assert e_mix <= 1e-6
+        assert e_plain <= 1e-4

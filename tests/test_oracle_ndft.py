@@ -74,3 +74,31 @@ def test_unit_trajectory_rescale():
     np.testing.assert_allclose(ndft.unit_trajectory(rad), [[0.25, -0.5]])
     unit = np.array([[0.25, 0.5]])
     np.testing.assert_allclose(ndft.unit_trajectory(unit), [[0.25, -0.5]])
+
+
+@pytest.mark.parametrize("shape", [(16,), (12, 10), (8, 6, 10)])
+def test_ndft_gpu_matches_oracle(shape):
+    """The separable FP64 NDFT used by the large-size GPU tests
+    (``tests/ndft_gpu.py``, run here on torch CPU tensors) equals the
+    oracle's explicit sums, full and at sample / voxel subsets."""
+    import torch
+
+    import ndft_gpu
+
+    d = len(shape)
+    k = truncnorm_traj(300, d, 11).astype(np.float64)
+    om = torch.from_numpy(2 * np.pi * k)
+    x = rand_cplx((2, *shape), 12).astype(np.complex128)
+    y = rand_cplx((2, len(k)), 13).astype(np.complex128)
+    want2 = np.stack([ndft.type2(k, x[t], shape) for t in range(2)])
+    want1 = np.stack([ndft.type1(k, y[t], shape) for t in range(2)])
+    got2 = ndft_gpu.type2(om, torch.from_numpy(x), shape, chunk=77).numpy()
+    got1 = ndft_gpu.type1(om, torch.from_numpy(y), shape, chunk=77).numpy()
+    np.testing.assert_allclose(got2, want2, rtol=1e-12, atol=1e-10)
+    np.testing.assert_allclose(got1, want1, rtol=1e-12, atol=1e-10)
+    idx = torch.tensor([0, 5, 299, 17])
+    np.testing.assert_allclose(ndft_gpu.type2_at(om, torch.from_numpy(x), shape, idx).numpy(),
+                               want2[:, idx.numpy()], rtol=1e-12, atol=1e-10)
+    vox = torch.tensor([0, 3, int(np.prod(shape)) - 1])
+    np.testing.assert_allclose(ndft_gpu.type1_at(om, torch.from_numpy(y), shape, vox, chunk=50).numpy(),
+                               want1.reshape(2, -1)[:, vox.numpy()], rtol=1e-12, atol=1e-10)
