@@ -121,6 +121,47 @@ def toeplitz_supported(shape):
     return all(smooth(2 * int(n)) and 4 * int(n) <= 2730 for n in shape)
 
 
+def stack_layout(samples, shape):
+    """(2D samples in [-0.5, 0.5), z positions) when a 3D trajectory is a
+    stack: NZ planes at Cartesian kz = (z - Z//2) / Z (``stacked2traj3d``),
+    each carrying the same 2D samples in the same order, plane after plane;
+    None otherwise (and for 2D shapes)."""
+    if len(shape) != 3:
+        return None
+    s = np.asarray(samples, dtype=np.float64).reshape(-1, 3)
+    if len(s) == 0:
+        return None
+    if np.max(np.abs(s)) - 1e-4 > 0.5:      # radians
+        s = s / (2 * np.pi)
+    Z = int(shape[2])
+    NZ = int(np.count_nonzero(np.diff(s[:, 2]))) + 1
+    if len(s) % NZ:
+        return None
+    planes = s.reshape(NZ, len(s) // NZ, 3)
+    if not (np.all(planes[:, :, 2] == planes[:, :1, 2])
+            and np.allclose(planes[:, :, :2], planes[:1, :, :2], rtol=0.0, atol=1e-7)):
+        return None
+    zf = planes[:, 0, 2] * Z + Z // 2
+    zi = np.rint(zf)
+    if np.max(np.abs(zf - zi)) > 1e-4 or zi.min() < 0 or zi.max() >= Z:
+        return None
+    return planes[0, :, :2].astype(np.float32), zi.astype(np.int64)
+
+
+def estimate_bytes_stacked(shape, n_samples2d, n_planes, n_coils, n_trans, toeplitz=False):
+    """Device memory of a stacked plan, its data and CG state (bytes)."""
+    X, Y, Z = (int(n) for n in shape)
+    N = float(X * Y * Z)
+    T2 = n_trans * n_planes                                   # 2D transforms per launch
+    grid2 = T2 * 4.0 * X * Y * 8                               # 2D fine grids
+    planes = T2 * float(X * Y) * 8
+    pts = n_samples2d * (4 + 16) + 4.5 * n_samples2d * 4
+    ksp = 3.0 * n_coils * n_planes * n_samples2d * 8
+    img = (n_coils + 6) * N * 8
+    gram = (T2 * 4.0 * X * Y + 4.0 * X * Y) * 8 + 4 * n_coils * N * 8 if toeplitz else 0
+    return int(grid2 + planes + pts + ksp + img + gram + (8 << 20))
+
+
 def plan_ntrans(shape, n_coils, upsampfac=2.0, cap_bytes=8 << 30):
     """Transforms per launch of a resident plan: enough for 16 coils (requests
     with more are chunked), within ``cap_bytes`` of fine grids."""
@@ -129,30 +170,75 @@ def plan_ntrans(shape, n_coils, upsampfac=2.0, cap_bytes=8 << 30):
 
 
 
+def stacked_ntrans(shape, n_planes, n_coils, cap_bytes=4 << 30):
+    """(batch, coil) pairs per launch of a resident stacked plan."""
+    pair = 8.0 * 5 * shape[0] * shape[1] * n_planes     # planes + 2D fine grids of one pair
+    return int(max(1, min(max(n_coils, 8), cap_bytes // pair)))
+
+
+def _own_lipschitz(op, seed, toeplitz, max_iter=10):
+    """Largest eigenvalue of the operator's own single-coil normal operator
+    from a seeded start vector.  (The reference's stacked operator returns
+    its 2D operator's constant, twice this one; the server uses the
+    constant of the operator it iterates with.)"""
+    from .base import power_method, random_start
+
+    smaps, C, B, sq = op.smaps, op.n_coils, op.n_batchs, op.squeeze_dims
+    op.smaps = None
+    op.n_coils, op.n_batchs, op.squeeze_dims = 1, 1, True
+    try:
+        x0 = random_start(op.shape, op.device, seed)
+        L, _ = power_method(max_iter, lambda v: op.gram_op(v, toeplitz=toeplitz),
+                            norm_func=lambda v: torch.linalg.vector_norm(v.reshape(-1)), x=x0)
+    finally:
+        op.n_coils, op.n_batchs, op.squeeze_dims = C, B, sq
+        op.smaps = smaps
+    return float(L)
+
+
 class _Plan:
     """A resident operator for one trajectory and image shape: sorted points,
     workspaces, the Lipschitz constant and the Toeplitz kernel, shared by every
     request of that protocol whatever its coil count (the coil count and maps
-    are set per request)."""
+    are set per request).  A 3D trajectory that is a stack of one 2D pattern
+    on Cartesian kz planes gets the stacked 2.5D operator (a 2D NUFFT per
+    plane and a z DFT: the same operator as the 3D NUFFT on those samples,
+    at half the fine-grid cells), whose Toeplitz form is the 2D one applied
+    slice by slice when every plane is sampled once."""
 
-    def __init__(self, key, samples, shape, n_coils, stream, seed, toeplitz):
-        nt = plan_ntrans(shape, n_coils)
+    def __init__(self, key, samples, shape, n_coils, stream, seed, toeplitz, layout=None):
+        from .stacked import MRIStackedHipNUFFT
+
         with torch.cuda.stream(stream):
-            # built for nt coils (its workspace holds nt transforms per launch);
-            # each request sets its own coil count, larger ones are chunked
-            self.op = MRIHipNUFFT(samples, shape, n_coils=max(nt, 1), smaps=None,
-                                  squeeze_dims=False)
-            self.op._smaps = torch.zeros((n_coils, *shape), dtype=torch.complex64,
-                                         device=self.op.device)
+            if layout is not None:
+                s2d, zi = layout
+                nt = stacked_ntrans(shape, len(zi), n_coils)
+                self.op = MRIStackedHipNUFFT(s2d, shape, n_coils=n_coils, z_index=zi,
+                                             n_trans=nt, squeeze_dims=False)
+                self.toeplitz = (bool(toeplitz) and self.op.toeplitz_exact
+                                 and toeplitz_supported(shape[:2]))
+                self.bytes = estimate_bytes_stacked(shape, len(s2d), len(zi),
+                                                    max(n_coils, nt), nt, self.toeplitz)
+            else:
+                nt = plan_ntrans(shape, n_coils)
+                # built for nt coils (its workspace holds nt transforms per
+                # launch); each request sets its own coil count, larger ones
+                # are chunked
+                self.op = MRIHipNUFFT(samples, shape, n_coils=max(nt, 1), smaps=None,
+                                      squeeze_dims=False)
+                self.toeplitz = bool(toeplitz) and toeplitz_supported(shape)
+                self.bytes = estimate_bytes(shape, self.op.n_samples,
+                                            max(n_coils, self.op.n_trans),
+                                            toeplitz=self.toeplitz)
             self.op.n_coils = n_coils
-            self.toeplitz = bool(toeplitz) and toeplitz_supported(shape)
+            self.op.smaps = torch.zeros((n_coils, *shape), dtype=torch.complex64,
+                                        device=self.op.device)
             if self.toeplitz:
                 self.op.compute_toeplitz_kernel()
             # the power iterations on the Toeplitz form when there is one
-            self.lipschitz = float(self.op.get_lipschitz_cst(seed=seed, toeplitz=self.toeplitz))
+            self.lipschitz = _own_lipschitz(self.op, seed, self.toeplitz)
         self.key = key
-        self.bytes = estimate_bytes(shape, self.op.n_samples, max(n_coils, self.op.n_trans),
-                                    toeplitz=self.toeplitz)
+        self.stacked = layout is not None
         self.in_use = False
 
     def bind(self, req, stream):
@@ -165,7 +251,7 @@ class _Plan:
                              f"{(C, *op.shape)}")
         with torch.cuda.stream(stream):
             op.n_coils = C
-            op._smaps = to_device(req.smaps, torch.complex64, op.device).reshape(C, *op.shape)
+            op.smaps = to_device(req.smaps, torch.complex64, op.device).reshape(C, *op.shape)
 
 
 class _CGJob:
@@ -275,10 +361,13 @@ class ReconServer:
     """Slot-based continuous batching under an HBM budget (see module doc)."""
 
     def __init__(self, hbm_budget_bytes=64 << 30, max_slots=8, use_graph=True, seed=1234,
-                 toeplitz=True, depth=1):
+                 toeplitz=True, depth=1, stacked=True):
         if not MRIHipNUFFT.available:
             raise RuntimeError("hipnufft is not available (library or GPU missing)")
         self.toeplitz = bool(toeplitz)
+        # stack-of-stars requests on the stacked 2.5D operator
+        self.stacked = bool(stacked)
+        self._layouts = {}           # plan key -> (stack layout or None, plan bytes)
         # CG iterations enqueued per active request per pass of the loop
         # (0: all of them at admission)
         self.depth = int(depth)
@@ -320,10 +409,7 @@ class ReconServer:
         self._next_id = max(self._next_id, req.rid) + 1
         try:
             validate_request(req)
-            C = int(req.kspace.shape[0])
-            need = estimate_bytes(req.shape, req.kspace.shape[1],
-                                  max(C, plan_ntrans(req.shape, C)),
-                                  toeplitz=self.toeplitz and toeplitz_supported(req.shape))
+            _, need = self._estimate(req)
             if need > self.budget:
                 raise ValueError(f"request {req.rid} needs {need} bytes, over the HBM budget "
                                  f"of {self.budget}")
@@ -332,6 +418,28 @@ class ReconServer:
             return req.rid
         self.waiting.append(req)
         return req.rid
+
+    def _estimate(self, req):
+        """(stack layout or None, device bytes of a new plan) for a request,
+        computed once per plan key."""
+        key = self._key(req)
+        hit = self._layouts.get(key)
+        if hit is None:
+            C = int(req.kspace.shape[0])
+            shape = tuple(int(n) for n in req.shape)
+            layout = stack_layout(req.samples, shape) if self.stacked else None
+            if layout is not None:
+                nt = stacked_ntrans(shape, len(layout[1]), C)
+                toep = self.toeplitz and toeplitz_supported(shape[:2]) and \
+                    len(layout[1]) == shape[2] and len(set(layout[1].tolist())) == shape[2]
+                need = estimate_bytes_stacked(shape, len(layout[0]), len(layout[1]),
+                                              max(C, nt), nt, toep)
+            else:
+                need = estimate_bytes(shape, req.kspace.shape[1],
+                                      max(C, plan_ntrans(shape, C)),
+                                      toeplitz=self.toeplitz and toeplitz_supported(shape))
+            hit = self._layouts[key] = (layout, need)
+        return hit
 
     def _fail(self, req, admitted, now, exc):
         arr = req.arrival if self._realtime else 0.0
@@ -356,19 +464,18 @@ class ReconServer:
         plan = self.plans.get(key)
         if plan is not None and plan.in_use:
             return False
-        if plan is None:
-            need = estimate_bytes(req.shape, req.kspace.shape[1],
-                                  max(C, plan_ntrans(req.shape, C)),
-                                  toeplitz=self.toeplitz and toeplitz_supported(req.shape))
-            if not self._make_room(need):
-                return False
+        layout, need = self._estimate(req)
+        if plan is None and not self._make_room(need):
+            return False
         si = self.free_streams.pop()
         stream = self.streams[si]
         built = plan is None
         try:
             if built:
-                traj = proper_trajectory(req.samples, normalize="pi")
-                plan = _Plan(key, traj, tuple(req.shape), C, stream, self.seed, self.toeplitz)
+                # a stacked plan takes its 2D samples from the layout
+                traj = None if layout is not None else proper_trajectory(req.samples, "pi")
+                plan = _Plan(key, traj, tuple(req.shape), C, stream, self.seed, self.toeplitz,
+                             layout)
                 self.plans[key] = plan
                 self.stats["plan_builds"] += 1
             else:
@@ -394,7 +501,7 @@ class ReconServer:
         self.free_streams.append(si)
         if plan is not None:
             plan.in_use = False
-        if drop_plan and self.plans.get(key) is plan:
+        if drop_plan and plan is not None and self.plans.get(key) is plan:
             del self.plans[key]
 
     def _retire(self, now):
@@ -503,11 +610,13 @@ def make_request(kind, rng, protocol_choices=True):
 
     radial2d: 256^2, Nc in 64..256 (steps of 64) x Ns 512, 1..32 coils;
     spiral2d: 384^2, Nc in 32..96 (steps of 32) x Ns 4096, 16 coils;
-    stars3d:  128^3 stack of stars, Nc 256 x Ns 256 x 128 partitions, 8 coils.
+    stars3d:  128^3 stack of stars, Nc 256 x Ns 256 on 128 Cartesian
+              partitions kz = (z - 64) / 128 (plane after plane), 8 coils.
     The k-space is the SENSE forward of a random image (so the solve has a
     known answer); requests of one protocol share a trajectory key.
     """
-    from .trajectories import gaussian_smaps, initialize_2D_radial, initialize_2D_spiral, stack
+    from .stacked import stacked2traj3d
+    from .trajectories import gaussian_smaps, initialize_2D_radial, initialize_2D_spiral
 
     if kind == "radial2d":
         Nc = int(rng.choice([64, 128, 192, 256])) if protocol_choices else int(rng.integers(64, 257))
@@ -524,7 +633,7 @@ def make_request(kind, rng, protocol_choices=True):
     elif kind == "stars3d":
         C = 8
         shape = (128, 128, 128)
-        traj = stack(initialize_2D_radial(256, 256), 128).reshape(-1, 3)
+        traj = stacked2traj3d(initialize_2D_radial(256, 256).reshape(-1, 2), np.arange(128), 128)
         key = ("stars3d",)
     else:
         raise ValueError(kind)

@@ -145,15 +145,28 @@ class MRIStackedHipNUFFT(FourierOperatorBase):
         self.dtype = np.float32
         self._stack = _capi.RawStack(self.shape, self.z_index)
         self._planes = None
+        self._gram = None
         self._smaps = None
-        if smaps is not None:
-            s = to_device(smaps, torch.complex64, self.device)
-            self._smaps = s.reshape(self.n_coils, *self.shape).contiguous()
+        self._smaps_zxy = None
+        self.smaps = smaps
 
     # ------------------------------------------------------------ properties
     @property
     def smaps(self):
         return self._smaps
+
+    @smaps.setter
+    def smaps(self, new_smaps):
+        """Coil maps (C, X, Y, Z), kept on the device; the coil count follows."""
+        self._smaps_zxy = None
+        if new_smaps is None:
+            self._smaps = None
+            return
+        s = to_device(new_smaps, torch.complex64, self.device)
+        if tuple(s.shape[1:]) != tuple(self.shape):
+            raise ValueError("Smaps shape does not match image shape.")
+        self.n_coils = s.shape[0]
+        self._smaps = s.contiguous()
 
     @property
     def density(self):
@@ -162,6 +175,7 @@ class MRIStackedHipNUFFT(FourierOperatorBase):
     @density.setter
     def density(self, d):
         self.operator.density = d
+        self._gram = None
 
     @property
     def uses_density(self):
@@ -195,6 +209,7 @@ class MRIStackedHipNUFFT(FourierOperatorBase):
         self.z_index = np.asarray(zi, dtype=np.int64)
         self._stack = _capi.RawStack(self.shape, self.z_index)
         self.operator.samples = self._samples2d
+        self._gram = None
 
     def _plane_buffer(self, T):
         NZ = len(self.z_index)
@@ -254,6 +269,70 @@ class MRIStackedHipNUFFT(FourierOperatorBase):
                 self._stack.adj(planes, self._smaps, out, C, t0, T, zscale)
         if img is not None and isinstance(img, torch.Tensor) and img.is_cuda:
             img.reshape(out.shape).copy_(out)
+        return from_device(self._safe_squeeze(out), kind)
+
+    # ---------------------------------------------------------------- Gram
+    @property
+    def toeplitz_exact(self):
+        """True when every z plane is sampled exactly once.  Then
+        ``A^H D A = norm^-2 S^H (I_z (x) A2^H D A2) S``: the unitary z
+        transform commutes with the 2D normal operator acting on each
+        z slice, so the Gram form is the 2D Toeplitz operator applied slice
+        by slice (no z transform at all)."""
+        Z = self.shape[-1]
+        return len(self.z_index) == Z and np.array_equal(np.sort(self.z_index), np.arange(Z))
+
+    def compute_toeplitz_kernel(self):
+        """The 2D Toeplitz kernel of the stack's normal operator (density
+        included); raises when the planes do not cover each z once."""
+        if not self.toeplitz_exact:
+            raise ValueError("the stacked Toeplitz form needs every z plane sampled exactly once")
+        op2 = self.operator
+        with torch.cuda.device(self.device):
+            self._gram = _capi.RawGram(op2.shape, op2.samples, op2.density, eps=op2.eps,
+                                       isign=-1, scale=self.inv_norm_factor ** 2,
+                                       n_trans=max(1, self.n_trans * len(self.z_index)))
+        return self._gram
+
+    def _maps_zxy(self):
+        """The coil maps as (C, Z, X, Y) z-slices, built once per set of maps."""
+        if self._smaps_zxy is None:
+            self._smaps_zxy = self._smaps.permute(0, 3, 1, 2).contiguous()
+        return self._smaps_zxy
+
+    def gram_op(self, data, toeplitz=True):
+        """``A^H D A x``: with ``toeplitz`` (and every plane sampled once) the
+        2D Toeplitz operator on each z slice of each coil image, expanded by
+        and combined with the coil maps; otherwise ``adj_op(op(x))``."""
+        self.check_shape(image=data)
+        if not (toeplitz and self.toeplitz_exact):
+            return self.adj_op(self.op(data))
+        kind = kind_of(data)
+        B, C = self.n_batchs, self.n_coils
+        X, Y, Z = self.shape
+        ic = 1 if self.uses_sense else C
+        with torch.cuda.device(self.device):
+            if self._gram is None:
+                self.compute_toeplitz_kernel()
+            x = to_device(data, torch.complex64, self.device).reshape(B, ic, X, Y, Z)
+            xz = x.permute(0, 1, 4, 2, 3)                             # (B, ic, Z, X, Y)
+            v = torch.empty((B, C, Z, X, Y), dtype=torch.complex64, device=self.device)
+            if self.uses_sense:
+                sz = self._maps_zxy()
+                torch.mul(xz, sz[None], out=v)                        # expand by the maps
+            else:
+                v.copy_(xz)
+            g = torch.empty_like(v)
+            n2 = B * C * Z
+            self._gram.apply(v.view(n2, X, Y), None, g.view(n2, X, Y), n2, 1)
+            if self.uses_sense:
+                # coil combine by elementwise passes (no reduction kernels,
+                # so the form can be captured in a HIP graph)
+                acc = torch.mul(g[:, 0:1], torch.conj(sz[0:1])[None])
+                for c in range(1, C):
+                    acc += g[:, c:c + 1] * torch.conj(sz[c:c + 1])[None]
+                g = acc
+            out = g.permute(0, 1, 3, 4, 2).contiguous()               # (B, ic, X, Y, Z)
         return from_device(self._safe_squeeze(out), kind)
 
     def get_lipschitz_cst(self, max_iter=10, **kwargs):

@@ -225,3 +225,30 @@ def test_toeplitz_support_rule():
     assert toeplitz_supported((192, 192, 192)) and toeplitz_supported((48, 48))
     assert not toeplitz_supported((40, 56))      # 112 = 2^4 * 7
     assert not toeplitz_supported((1024, 1024))  # kernel build grid 4096 > 2730
+
+
+def test_stack_layout_detection():
+    """The server recognises a stack of one 2D pattern on Cartesian kz
+    planes (in unit or radian coordinates) and nothing else."""
+    import numpy as np
+
+    from hipnufft import stacked2traj3d
+    from hipnufft.serving import stack_layout
+
+    s2 = np.random.default_rng(0).uniform(-0.5, 0.5, (50, 2)).astype(np.float32)
+    zi = np.array([3, 0, 7, 5])
+    t3 = stacked2traj3d(s2, zi, 8)
+    got = stack_layout(t3, (16, 16, 8))
+    assert got is not None
+    np.testing.assert_array_equal(got[1], zi)
+    np.testing.assert_allclose(got[0], s2, atol=1e-7)
+    rad = (t3 * np.float32(2 * np.pi)).astype(np.float32)
+    np.testing.assert_array_equal(stack_layout(rad, (16, 16, 8))[1], zi)
+    assert stack_layout(t3[:, :2], (16, 16)) is None                 # 2D
+    shifted = t3.copy()
+    shifted[:, 2] += 0.5 / 8                                         # half-integer kz
+    assert stack_layout(shifted, (16, 16, 8)) is None
+    jitter = t3.copy()
+    jitter[60, 0] += 1e-3                                            # plane 2 differs
+    assert stack_layout(jitter, (16, 16, 8)) is None
+    assert stack_layout(np.random.default_rng(1).uniform(-0.5, 0.5, (200, 3)), (8, 8, 8)) is None
