@@ -219,6 +219,20 @@ int hipnufft_dot_c64_rows(const void* a_d, const void* b_d, int64_t rows,
                           int64_t n, int conj, double* out_d, double* ws_d,
                           void* stream);
 
+/* hipnufft_dot_c64_rows that allocates nothing: no hipMalloc / hipFree and
+ * no caller workspace; the partial sums live in a static device array of
+ * the library (one per GPU, 1 MiB, kept until the process ends), so rows <=
+ * HIPNUFFT_NOALLOC_DOT_ROWS (HIPNUFFT_ERR_ARG otherwise).  Same kernels and
+ * summation order: each row equals hipnufft_dot_c64_rows bit for bit.  Every
+ * call on one GPU shares that array, so calls that overlap on one GPU (two
+ * streams at once) race; calls in one stream do not.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid allocation). */
+#define HIPNUFFT_NOALLOC_DOT_ROWS 64
+int hipnufft_dot_c64_rows_noalloc(const void* a_d, const void* b_d,
+                                  int64_t rows, int64_t n, int conj,
+                                  double* out_d, void* stream);
+
 /* Sorted-point introspection for bit-exact tests: copies the permutation
  * (sorted position -> original index, M int32), the bin offsets
  * (nkeys+1 int32) and the spread lists (entries int32 and offsets

@@ -6,11 +6,18 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
 
 #include "internal.h"
+
+// Partial sums of hipnufft_dot_c64_rows_noalloc: HIPNUFFT_NOALLOC_DOT_ROWS
+// rows of HIPNUFFT_DOT_WS / 2 double2 each (1 MiB).  Every GPU the library
+// runs on holds its own copy until the process ends; host code reaches it
+// only through the per-device address resolved by noalloc_scratch().
+static __device__ __align__(256) double2 scratch[HIPNUFFT_NOALLOC_DOT_ROWS * (HIPNUFFT_DOT_WS / 2)];
 
 namespace hn {
 
@@ -371,6 +378,38 @@ int dot_c64(const float2* a, const float2* b, int64_t rows, int64_t n, int conj,
   if (!ws) HN_HIP(hipFreeAsync(part, s));
   return 0;
 }
+
+namespace {
+
+std::mutex g_scratch_mu;
+std::vector<double2*> g_scratch_addr;  // device address of `scratch`, per device
+
+// Device address of `scratch` on the device of stream s, resolved once per
+// device with that device current (a symbol's host-side name is not a
+// device address).
+int noalloc_scratch(hipStream_t s, double2** out) {
+  int dev = 0;
+  HN_HIP(hipGetDevice(&dev));
+  if (s) HN_HIP(hipStreamGetDevice(s, &dev));
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if ((int)g_scratch_addr.size() <= dev) g_scratch_addr.resize(dev + 1, nullptr);
+  if (!g_scratch_addr[dev]) {
+    int cur = 0;
+    HN_HIP(hipGetDevice(&cur));
+    if (cur != dev) HN_HIP(hipSetDevice(dev));
+    void* addr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&addr, HIP_SYMBOL(scratch));
+    if (cur != dev) HN_HIP(hipSetDevice(cur));
+    if (e != hipSuccess || !addr)
+      return set_error(HIPNUFFT_ERR_HIP,
+                       std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_addr[dev] = (double2*)addr;
+  }
+  *out = g_scratch_addr[dev];
+  return 0;
+}
+
+}  // namespace
 
 int create_fft_plan(int ndim, const int64_t* nmodes, const int64_t* nf, int ntrans, hipStream_t s,
                     hipnufft_plan_s** out) {
@@ -813,6 +852,20 @@ int hipnufft_dot_c64_rows(const void* a, const void* b, int64_t rows, int64_t n,
   if (!a || !b || !out) return set_error(HIPNUFFT_ERR_ARG, "null pointer");
   if (n < 0 || rows < 0) return set_error(HIPNUFFT_ERR_ARG, "negative size");
   return dot_c64((const float2*)a, (const float2*)b, rows, n, conj, out, ws, (hipStream_t)stream);
+}
+
+int hipnufft_dot_c64_rows_noalloc(const void* a, const void* b, int64_t rows, int64_t n,
+                                  int conj, double* out, void* stream) {
+  if (!a || !b || !out) return set_error(HIPNUFFT_ERR_ARG, "null pointer");
+  if (n < 0 || rows < 0) return set_error(HIPNUFFT_ERR_ARG, "negative size");
+  if (rows > HIPNUFFT_NOALLOC_DOT_ROWS)
+    return set_error(HIPNUFFT_ERR_ARG, "dot_c64_rows_noalloc: at most " +
+                                           std::to_string(HIPNUFFT_NOALLOC_DOT_ROWS) + " rows");
+  double2* ws = nullptr;
+  int rc = noalloc_scratch((hipStream_t)stream, &ws);
+  if (rc) return rc;
+  return dot_c64((const float2*)a, (const float2*)b, rows, n, conj, out, (double*)ws,
+                 (hipStream_t)stream);
 }
 
 int hipnufft_get_sort(const hipnufft_plan* p, int32_t* perm, int32_t* bin_start, int32_t* list,

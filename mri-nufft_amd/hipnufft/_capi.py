@@ -74,6 +74,7 @@ _SIGNATURES = {
     "hipnufft_sense_grad_traj": [_P, _P, _P, _P, _I, _I, _F, _I, _I, _P],
     "hipnufft_dot_c64": [_P, _P, _I64, _I, _P, _P, _P],
     "hipnufft_dot_c64_rows": [_P, _P, _I64, _I64, _I, _P, _P, _P],
+    "hipnufft_dot_c64_rows_noalloc": [_P, _P, _I64, _I64, _I, _P, _P],
     "hipnufft_get_sort": [_P, _P, _P, _P, _P, ctypes.POINTER(_I64)],
     "hipnufft_plan_info": [_P, ctypes.POINTER(_I64), ctypes.POINTER(_I),
                            ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_I),
@@ -505,4 +506,24 @@ def dot_c64_rows(a, b, conj=True, out=None, ws=None):
         out = torch.empty((R, 2), dtype=torch.float64, device=a.device)
     check(lib().hipnufft_dot_c64_rows(ptr(a), ptr(b), R, n, 1 if conj else 0, ptr(out), ptr(ws),
                                       current_stream()), "dot_rows")
+    return out
+
+
+NOALLOC_DOT_ROWS = 64
+
+
+def dot_c64_rows_noalloc(a, b, conj=True, out=None):
+    """``dot_c64_rows`` through the entry point that allocates nothing (its
+    partial sums live in the library's static device array): at most
+    NOALLOC_DOT_ROWS rows; concurrent calls on one GPU from different
+    streams race.  ``out`` must be given by callers that cannot allocate."""
+    a = a.resolve_conj().resolve_neg().contiguous()
+    b = b.resolve_conj().resolve_neg().contiguous()
+    if a.shape != b.shape or a.ndim != 2:
+        raise ValueError("dot_c64_rows_noalloc: need two (rows, n) tensors of the same shape")
+    R, n = a.shape
+    if out is None:
+        out = torch.empty((R, 2), dtype=torch.float64, device=a.device)
+    check(lib().hipnufft_dot_c64_rows_noalloc(ptr(a), ptr(b), R, n, 1 if conj else 0, ptr(out),
+                                              current_stream()), "dot_rows_noalloc")
     return out
