@@ -210,6 +210,7 @@ void free_plan(hipnufft_plan_s* p) {
   (void)hipFree(p->set_off);
   (void)hipFree(p->sub_tile);
   (void)hipFree(p->sub_begin);
+  (void)hipFree(p->sub_mask);
   (void)hipFree(p->list_pts);
   (void)hipFree(p->list_start);
   (void)hipFree(p->it_tile);

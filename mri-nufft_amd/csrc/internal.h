@@ -150,6 +150,11 @@ struct hipnufft_plan_s {
   int32_t* sub_tile = nullptr;    // interp work items
   int32_t* sub_begin = nullptr;
   int64_t nsub = 0;
+  // per interp work item, per row of its staged region: the 16-cell chunks
+  // along the last axis its points read (2D / 3D; mask_rows = 0: none)
+  uint16_t* sub_mask = nullptr;
+  int64_t sub_mask_cap = 0;
+  int mask_rows = 0;
   int32_t* list_pts = nullptr;    // spread lists (sorted-point positions)
   int32_t* list_start = nullptr;  // nsets * nbins + 1
   int64_t nlist = 0;
@@ -268,6 +273,10 @@ int launch_spread_sorted(const hipnufft_plan_s* p, const float2* in_sorted,
                          SetRange r, float2* grid, int64_t grid_stride);
 // Trajectory gradient by the kernel derivative (type-2 direction); dy has
 // the k-space layout of SetRange.
+// The chunk masks of the interpolation work items' regions (sub_mask),
+// built once per setpts from the sorted points; leaves mask_rows = 0 (and
+// interpolation staging whole regions) where they do not apply.
+int build_interp_masks(hipnufft_plan_s* p);
 int launch_grad_traj(const hipnufft_plan_s* p, const float2* grid,
                      int64_t grid_stride, SetRange r, const float2* dy,
                      float scale, float* gtraj);

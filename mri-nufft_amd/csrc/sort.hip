@@ -646,6 +646,7 @@ int build_points(hipnufft_plan_s* p, int nsets, const int64_t* Ms, const float* 
     HN_HIP(hipMemsetAsync(p->list_start, 0, sizeof(int32_t) * (nkeys + 1), s));
     p->nsub = 0;
     p->nlist = 0;
+    p->mask_rows = 0;
     p->sub_of_set.assign(nsets + 1, 0);
     int rc0 = build_spread_items(p);  // empty tiles still write zeros
     if (rc0) return rc0;
@@ -745,6 +746,7 @@ int build_points(hipnufft_plan_s* p, int nsets, const int64_t* Ms, const float* 
                        p->sub_begin);
     HN_LAUNCH_CHECK();
   }
+  if ((rc = build_interp_masks(p))) return rc;
 
   // spread lists: one entry per (point, touched tile), stably sorted by tile
   {

@@ -89,7 +89,15 @@ struct InterpArgs {
   int stage_lanes, stage_rows;
   uint32_t lane_mag;
   int span;  // bins along the last axis per work item (region: span tiles + halo)
+  int stage_chunks;  // 16-cell chunks a region row of the last axis can meet
+  int mask_stage;    // stage only the chunks the points read (needs grid_bytes)
+  int grid_bytes;    // bytes of one transform's grid, < 2^31
+  const uint16_t* sub_mask;  // per work item and region row: the chunks read
 };
+
+// dword 3 of a raw buffer resource on gfx9-family GPUs (gfx950 included):
+// 32-bit data format, no swizzle, out-of-range offsets return zero
+constexpr int kBufferWord3 = 0x00020000;
 
 __device__ __forceinline__ int mdiv(int c, uint32_t magic) {
   return (int)__umulhi((uint32_t)c, magic);
@@ -738,6 +746,190 @@ __global__ __launch_bounds__(4 * 64 * GH, 4 / GH) void k_spread_mfma(SpreadArgs 
   }
 }
 
+// Record of an entry for the few-transform matrix kernel: as
+// write_mfma_entry (GT strengths, each axis's weight at every tile cell),
+// with the touched-block mask over blocks of 4 x 4 x ZB cells.
+template <int GT, int ZB>
+__device__ __forceinline__ void write_mfmaz_entry(float* sp, const BlkPos& r,
+                                                  const SpreadArgs& a, int t0, int Gv,
+                                                  const int* td, const int* ext, const int* nf,
+                                                  const int* nb, float inv_hw, float beta) {
+  const int W = a.g.w;
+  float2 v[GT];
+#pragma unroll
+  for (int g = 0; g < GT; ++g)
+    v[g] = g < Gv ? a.in[(int64_t)(t0 + g) * a.in_stride + r.p] : make_float2(0.f, 0.f);
+  uint32_t hit[3] = {0u, 0u, 0u};
+  float* wr = sp + kBlkHead + 2 * GT;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const int edge = d == 2 ? ZB : kBlkE;
+    for (int x = 0; x < td[d]; ++x) {
+      int j = x - r.st[d];
+      if (j < 0) j += nf[d];
+      if (j >= nf[d]) j -= nf[d];
+      float w = 0.0f;
+      if (x < ext[d] && j < W) {
+        w = es_eval(r.off[d] - (float)j, inv_hw, beta);
+        hit[d] |= 1u << (x / edge);
+      }
+      wr[x] = w;
+    }
+    wr += td[d];
+  }
+  uint32_t mask = 0u;
+  for (int b0 = 0; b0 < nb[0]; ++b0)
+    for (int b1 = 0; b1 < nb[1]; ++b1)
+      for (int b2 = 0; b2 < nb[2]; ++b2)
+        if ((hit[0] >> b0) & (hit[1] >> b1) & (hit[2] >> b2) & 1u)
+          mask |= 1u << ((b0 * nb[1] + b1) * nb[2] + b2);
+  reinterpret_cast<uint32_t*>(sp)[0] = mask;
+  float* sv = sp + kBlkHead;
+#pragma unroll
+  for (int g = 0; g < GT; ++g) {
+    sv[2 * g] = v[g].x;
+    sv[2 * g + 1] = v[g].y;
+  }
+}
+
+// Matrix-core spreading for GT = 1, 2 or 4 transforms per workgroup.  The
+// 8-transform kernel above puts transforms x re / im in the 16 columns of
+// each product, so with fewer transforms most of its products are zero.
+// Here the columns are (z cell, transform, re / im) instead: for a block of
+// 4 x 4 x ZB cells (ZB = 8 / GT) and four entries touching it, the block's
+// sums are one v_mfma_f32_16x16x4f32 of A (16 xy cells of the block x 4
+// entries: the products wx wy) by B (4 entries x 16 columns: wz times the
+// strength).  Lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16] and
+// holds D[4 (l / 16) + r][l % 16]: cells (x = l / 16, y = r) of column
+// l % 16.  A wave owns blocks wave + 4 i and takes the entries touching each
+// four at a time in list order: deterministic, equal to the other kernels
+// to rounding.
+template <int GT>
+__global__ __launch_bounds__(kThreads) void k_spread_mfmaz(SpreadArgs a) {
+  constexpr int ZB = 8 / GT;   // block edge along the last axis
+  constexpr int BPW = 8;       // blocks per wave (32 per tile at most)
+  extern __shared__ float smem[];
+  const int PW = mfma_record_floats(GT, a.g.tile[0] + a.g.tile[1] + a.g.tile[2]);
+  const int item = a.item0 + blockIdx.x;
+  const int key = a.it_tile[item];
+  const int set = key / a.g.nbins;
+  const int tile = key - set * a.g.nbins;
+  const int slot = a.it_slot[item];
+  const int t0 = blockIdx.y * GT;
+  const int tb = (set - a.k0) * a.ts;
+  const int Gv = min(GT, a.ts - t0);
+  const float inv_hw = 2.0f / a.g.w, beta = a.g.beta;
+
+  int tc[3], org[3], ext[3], td[3], nf[3], nb[3];
+  tile_coords<3>(tile, a.g.nt, tc);
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    td[d] = a.g.tile[d];
+    nf[d] = a.g.nf[d];
+    org[d] = tc[d] * td[d];
+    ext[d] = min(td[d], nf[d] - org[d]);
+    nb[d] = td[d] / (d == 2 ? ZB : kBlkE);
+  }
+  const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x / 64);
+  const int lane = (int)threadIdx.x & 63;
+  const int kq = lane >> 4;                 // entry of the group (A column, B row)
+  const int ai = lane & 15;
+  const int ax = ai >> 2, ay = ai & 3;      // A row: cell (ax, ay) of the block's xy face
+  const int bz = ai / (2 * GT);             // B column: z cell, transform, re / im
+  const int bg = (ai >> 1) % GT;
+  const int bri = ai & 1;
+  const int nblk = nb[0] * nb[1] * nb[2];
+  int bidx[BPW][3];
+  uint32_t bbit[BPW];
+#pragma unroll
+  for (int bi = 0; bi < BPW; ++bi) {
+    const int b = wave + 4 * bi;
+    bidx[bi][2] = b % nb[2];
+    bidx[bi][1] = (b / nb[2]) % nb[1];
+    bidx[bi][0] = b / (nb[2] * nb[1]);
+    bool live = b < nblk;
+    live = live && bidx[bi][0] * kBlkE < ext[0] && bidx[bi][1] * kBlkE < ext[1] &&
+           bidx[bi][2] * ZB < ext[2];
+    bbit[bi] = live ? 1u << b : 0u;
+  }
+
+  f32x4 acc[BPW];
+#pragma unroll
+  for (int bi = 0; bi < BPW; ++bi) acc[bi] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+  const int e0 = a.it_begin[item];
+  const int e1 = min(e0 + a.smax, a.list_start[key + 1]);
+  const int q = (int)threadIdx.x;
+  for (int eb = e0; eb < e1; eb += kThreads) {
+    const int n = min(kThreads, e1 - eb);
+    if (q < n) {
+      BlkPos r;
+      load_blk_pos(a, a.list_pts[eb + q], org, r);
+      write_mfmaz_entry<GT, ZB>(smem + q * PW, r, a, t0, Gv, td, ext, nf, nb, inv_hw, beta);
+    }
+    __syncthreads();
+    for (int base = 0; base < n; base += 64) {
+      const int qi = base + lane;
+      const uint32_t hm = qi < n ? reinterpret_cast<const uint32_t*>(smem + qi * PW)[0] : 0u;
+#pragma unroll
+      for (int bi = 0; bi < BPW; ++bi) {
+        if (!bbit[bi]) continue;  // uniform
+        uint64_t todo = __ballot((hm & bbit[bi]) != 0u);
+        while (todo) {  // uniform: four entries per step, list order
+          int ek[4];
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            ek[k] = todo ? __builtin_ctzll(todo) : -1;
+            todo &= todo - 1;
+          }
+          const int e = kq == 0 ? ek[0] : kq == 1 ? ek[1] : kq == 2 ? ek[2] : ek[3];
+          float av = 0.f, bv = 0.f;
+          if (e >= 0) {
+            const float* sp = smem + (base + e) * PW;
+            const float* w = sp + kBlkHead + 2 * GT;
+            av = w[kBlkE * bidx[bi][0] + ax] * w[td[0] + kBlkE * bidx[bi][1] + ay];
+            bv = w[td[0] + td[1] + ZB * bidx[bi][2] + bz] * sp[kBlkHead + 2 * bg + bri];
+          }
+          acc[bi] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc[bi], 0, 0, 0);
+        }
+      }
+    }
+    __syncthreads();
+  }
+
+  // lane l holds component bri of transform bg at cells (x = l / 16, y = r,
+  // z = bz); lanes 2k and 2k + 1 (re and im of one column) swap halves so
+  // that each writes two whole complex cells
+  const int TC = td[0] * td[1] * td[2];
+  const bool odd = lane & 1;
+  if (bg >= Gv) return;
+#pragma unroll
+  for (int bi = 0; bi < BPW; ++bi) {
+    if (!bbit[bi]) continue;
+    float2 out[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const float mine = odd ? acc[bi][2 + h] : acc[bi][h];
+      const float give = odd ? acc[bi][h] : acc[bi][2 + h];
+      const float other = __shfl_xor(give, 1);
+      out[h] = odd ? make_float2(other, mine) : make_float2(mine, other);
+    }
+    const int c0 = bidx[bi][0] * kBlkE + kq, c2 = bidx[bi][2] * ZB + bz;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int c1 = bidx[bi][1] * kBlkE + (odd ? 2 : 0) + h;
+      if (c0 >= ext[0] || c1 >= ext[1] || c2 >= ext[2]) continue;
+      if (slot >= 0) {
+        const int r = (c0 * td[1] + c1) * td[2] + c2;
+        a.part[((int64_t)slot * a.ntrans + tb + t0 + bg) * TC + r] = out[h];
+      } else {
+        const int64_t gi = ((int64_t)(org[0] + c0) * nf[1] + org[1] + c1) * nf[2] + org[2] + c2;
+        a.grid[(int64_t)(tb + t0 + bg) * a.grid_stride + gi] = out[h];
+      }
+    }
+  }
+}
+
 // Column-owner spreading for 3D tiles of 8 x 8 x 16 cells and 16 transforms
 // per workgroup: lane c of every wave owns tile column c (16 cells along
 // the contiguous axis), wave v owns transforms [4v, 4v + 4).  Per staged
@@ -1032,6 +1224,165 @@ __device__ __forceinline__ int span_bins(const InterpArgs& a, int tile) {
   return min(a.span, nl - tile % nl);
 }
 
+// Region geometry of an interpolation work item led by `tile`: origin and
+// extent per axis (span tiles along the last axis plus the kernel halo).
+template <int D, int W>
+__device__ __forceinline__ int region_of(const InterpArgs& a, int tile, int* rorg, int* R) {
+  int tc[3];
+  tile_coords<D>(tile, a.g.nt, tc);
+  int RC = 1;
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    R[d] = (d == D - 1 ? a.span : 1) * a.g.tile[d] + W;
+    rorg[d] = tc[d] * a.g.tile[d] - W / 2;
+    RC *= R[d];
+  }
+  return RC;
+}
+
+// Chunk masks of the work items' region rows (2D / 3D), built once per
+// setpts.  A point reads the region rows (l0 + j0 [, l1 + j1]), j < W, and
+// along the last axis the cells [l_last, l_last + W).  The last axis is cut
+// into 16-cell chunks aligned with the grid's 128-byte lines (region cell z
+// lies in chunk (z + phase) / 16, phase = the region origin mod 16), so a
+// row's mask is the OR, over the points whose footprint covers the row, of
+// the chunks their cells touch.  Without per-row atomics: each point ORs
+// its chunks into the entry of its footprint's first row, then separable
+// passes dilate the entries by W rows along each of the other axes.  One
+// workgroup per work item; out[sub * NR + r] for the NR region rows.
+template <int D, int W>
+__global__ __launch_bounds__(kThreads) void k_interp_masks(InterpArgs a, uint16_t* out) {
+  extern __shared__ uint32_t msk[];
+  constexpr int DL = D - 1;
+  const int sub = blockIdx.x;
+  const int key = a.sub_tile[sub];
+  const int set = key / a.g.nbins;
+  const int tile = key - set * a.g.nbins;
+  const int pb = a.sub_begin[sub];
+  const int pe = min(pb + a.max_sub, a.bin_start[key + span_bins<D>(a, tile)]);
+  int rorg[3], R[3];
+  region_of<D, W>(a, tile, rorg, R);
+  const int RB = D == 2 ? 1 : R[1];
+  const int NR = D == 2 ? R[0] : R[0] * R[1];
+  const int phase = ((rorg[DL] % 16) + 16) % 16;
+  uint32_t* first = msk;
+  uint32_t* dil = msk + NR;
+  for (int r = threadIdx.x; r < NR; r += kThreads) first[r] = 0u;
+  __syncthreads();
+  for (int p = pb + threadIdx.x; p < pe; p += kThreads) {
+    int l[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {  // the footprint origin of point_setup
+      int v = a.g.cell[d][p] - rorg[d];
+      if (v < 0) v += a.g.nf[d];
+      if (v >= a.g.nf[d]) v -= a.g.nf[d];
+      l[d] = min(max(v, 0), (d == DL ? a.span : 1) * a.g.tile[d]);
+    }
+    const int k0 = (l[DL] + phase) >> 4;
+    const int k1 = min((l[DL] + W - 1 + phase) >> 4, 15);
+    const uint32_t bits = (2u << k1) - (1u << k0);
+    atomicOr(&first[D == 2 ? l[0] : l[0] * R[1] + l[1]], bits);
+  }
+  __syncthreads();
+  for (int r = threadIdx.x; r < NR; r += kThreads) {  // covered along axis 0
+    const int ra = D == 2 ? r : mdiv(r, a.rmag[1]);
+    const int rb = r - ra * RB;
+    uint32_t m = 0u;
+    for (int i = 0; i < W && i <= ra; ++i) m |= first[(ra - i) * RB + rb];
+    dil[r] = m;
+  }
+  __syncthreads();
+  for (int r = threadIdx.x; r < NR; r += kThreads) {  // and along axis 1
+    uint32_t m = dil[r];
+    if constexpr (D == 3) {
+      const int ra = mdiv(r, a.rmag[1]);
+      const int rb = r - ra * RB;
+      m = 0u;
+      for (int j = 0; j < W && j <= rb; ++j) m |= dil[ra * RB + rb - j];
+    }
+    out[(int64_t)sub * NR + r] = (uint16_t)m;
+  }
+}
+
+// Staging of only the chunks the work item's points read (the masks of
+// k_interp_masks).  From the item's row masks a table of NC words per row is
+// built after the G regions: the row's wrapped grid offset for a needed
+// chunk, -1 otherwise, so each staged load reads one table word, as the
+// whole-region staging reads one row offset.  A cell of an unneeded chunk
+// loads nothing (a buffer load past the grid's end returns zero) and stores
+// that zero where no point reads.  The caller synchronises after.
+template <int D, int W, int NT>
+__device__ __forceinline__ void stage_region_masked(const InterpArgs& a, float2* lds, int tile,
+                                                    int sub, int t0, int G, int* rorg, int* R,
+                                                    int& RC) {
+  static_assert(D >= 2, "masked staging is for rows of 2D / 3D regions");
+  constexpr int kStageBatch = 20;  // 24 spills with the chunk table
+  constexpr int DL = D - 1;
+  RC = region_of<D, W>(a, tile, rorg, R);
+  const int RL = R[DL];
+  const int NR = D == 2 ? R[0] : R[0] * R[1];
+  const int NC = a.stage_chunks;
+  const int phase = ((rorg[DL] % 16) + 16) % 16;
+  int* table = reinterpret_cast<int*>(lds + G * RC);
+  const uint16_t* mask = a.sub_mask + (int64_t)sub * NR;
+  for (int r = threadIdx.x; r < NR; r += NT) {
+    int off;
+    if constexpr (D == 2) {
+      off = wrap_coord(rorg[0] + r, a.g.nf[0], a.wrap1);
+    } else {
+      const int l0 = mdiv(r, a.rmag[1]);
+      const int l1 = r - l0 * R[1];
+      off = wrap_coord(rorg[0] + l0, a.g.nf[0], a.wrap1) * a.g.nf[1] +
+            wrap_coord(rorg[1] + l1, a.g.nf[1], a.wrap1);
+    }
+    off *= a.g.nf[DL];
+    const uint32_t m = mask[r];
+    for (int k = 0; k < NC; ++k) table[r * NC + k] = ((m >> k) & 1u) ? off : -1;
+  }
+  __syncthreads();
+  const int L = a.stage_lanes, P = a.stage_rows;
+  const int rs = mdiv((int)threadIdx.x, a.lane_mag);
+  const int lz = (int)threadIdx.x - rs * L;
+  if (rs >= P) return;
+  for (int gg = 0; gg < G; ++gg) {
+    // the transform's grid as a raw buffer (every cell offset < 2^28, set
+    // by the host): an offset past its end returns zero without a memory
+    // access, which is how a cell of an unneeded chunk skips its load while
+    // every load of a batch stays one unconditional instruction
+    const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)(a.grid + (int64_t)(t0 + gg) * a.grid_stride), (short)0, a.grid_bytes,
+        kBufferWord3);
+    float2* dst = lds + gg * RC;
+    for (int z = lz; z < RL; z += L) {
+      const int xz = wrap_coord(rorg[DL] + z, a.g.nf[DL], a.wrap1);
+      const int kz = (z + phase) >> 4;
+      for (int r0 = rs; r0 < NR; r0 += kStageBatch * P) {
+        float2 v[kStageBatch];
+#pragma unroll
+        for (int u = 0; u < kStageBatch; ++u) {
+          const int r = r0 + u * P;
+          if (r < NR) {
+            const int t = table[r * NC + kz];
+            const uint32_t boff = t >= 0 ? (uint32_t)(t + xz) * 8u : 0x80000000u;
+            // two 32-bit buffer loads, merged into one 64-bit load by the
+            // compiler (its 64-bit buffer-load builtin lowers to a single
+            // dword here)
+            const uint32_t re = __builtin_amdgcn_raw_buffer_load_b32(rsrc, boff, 0, 0);
+            const uint32_t im = __builtin_amdgcn_raw_buffer_load_b32(rsrc, boff + 4u, 0, 0);
+            v[u] = make_float2(__builtin_bit_cast(float, re), __builtin_bit_cast(float, im));
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < kStageBatch; ++u) {
+          const int r = r0 + u * P;
+          if (r < NR) dst[r * RL + z] = v[u];
+        }
+      }
+    }
+  }
+}
+
+
 template <int D, int W>
 __device__ __forceinline__ void point_setup(const InterpArgs& a, int p, const int* rorg,
                                             int* ls, float (*wt)[W]) {
@@ -1050,10 +1401,40 @@ __device__ __forceinline__ void point_setup(const InterpArgs& a, int p, const in
   }
 }
 
-template <int D, int W, int NT>
-__global__ __launch_bounds__(NT) void k_interp(InterpArgs a) {
+// Work items of one transform are laid out along blockIdx.x in tile order
+// (last axis fastest).  Consecutive ids are dispatched to different XCDs, so
+// the halo rows that neighbouring items share would be fetched into two
+// L2s; the XCD remap gives each XCD a contiguous range of items (a slab of
+// the grid), where neighbours along the last two axes run close in time on
+// the same L2.  All XCDs still work on the same transform at once.
+// Writes one interpolated value (scale, and in DC mode the residual and
+// density) for point p, transform t0 + gg of the launch.
+__device__ __forceinline__ void interp_store(const InterpArgs& a, int set, int t, int p, int idx,
+                                             float dens, float sx, float sy) {
+  const int64_t ko = ksp_index(a.g, a.ts, set, t, idx);
+  float2 r = make_float2(a.scale * sx, a.scale * sy);
+  if (a.mode == 1) {
+    // residual for the spreading that follows: written in sorted order
+    const float2 y = a.obs[ko];
+    r.x = (r.x - y.x) * dens;
+    r.y = (r.y - y.y) * dens;
+    a.out[(int64_t)t * a.g.M + p] = r;
+  } else {
+    a.out[ko] = r;
+  }
+}
+
+// Interpolation of one work item: stage its region (whole, or with MASKED
+// only the chunks its points read), then gather each point's W^D footprint.
+// In 3D a work item holds about a hundred points for 512 threads, so the
+// points are spread over lp = 1, 2 or 4 consecutive lanes each (as many as
+// the item allows, uniform per workgroup): lane s of a point sums the kernel
+// rows j0 = s, s + lp, ..., and the partial sums meet by lane exchanges in
+// a fixed order (deterministic for a given trajectory).
+template <int D, int W, int NT, bool MASKED>
+__global__ __launch_bounds__(NT, 4) void k_interp(InterpArgs a) {
   extern __shared__ float2 lds[];
-  const int sub = a.sub0 + blockIdx.x;
+  const int sub = a.sub0 + (int)xcd_remap(blockIdx.x, gridDim.x);
   const int key = a.sub_tile[sub];
   const int set = key / a.g.nbins;
   const int tile = key - set * a.g.nbins;
@@ -1063,8 +1444,69 @@ __global__ __launch_bounds__(NT) void k_interp(InterpArgs a) {
   const int G = min(a.G, a.ts - t0);
   const int tb = (set - a.k0) * a.ts;    // chunk transform of the set's first
   int rorg[3], R[3], RC;
-  stage_region<D, W, NT>(a, lds, tile, tb + t0, G, rorg, R, RC);
+  if constexpr (MASKED && D >= 2) {
+    stage_region_masked<D, W, NT>(a, lds, tile, sub, tb + t0, G, rorg, R, RC);
+  } else {
+    stage_region<D, W, NT>(a, lds, tile, tb + t0, G, rorg, R, RC);
+  }
   __syncthreads();
+
+  if constexpr (D == 3) {
+    const int n = pe - pb;
+    int lg = 0;  // lp = 1 << lg lanes per point
+    while (lg < 2 && (n << (lg + 1)) <= NT) ++lg;
+    const int lp = 1 << lg;
+    const int trips = ((n << lg) + NT - 1) / NT;   // the same for every lane
+    const float inv_hw = 2.0f / W;
+    for (int it = 0; it < trips; ++it) {
+      const int q = it * NT + (int)threadIdx.x;
+      const int p = pb + (q >> lg);
+      const int s = q & (lp - 1);
+      const bool live = p < pe;
+      int ls[3] = {0, 0, 0};
+      float wt[D][W];
+      int idx = 0;
+      float dens = 1.0f, off0 = 0.0f;
+      if (live) {
+        point_setup<D, W>(a, p, rorg, ls, wt);
+        idx = a.g.idx[p];
+        dens = (a.mode == 1 && a.density) ? a.density[idx] : 1.0f;
+        off0 = a.g.off[0][p];
+      }
+      for (int gg = 0; gg < G; ++gg) {
+        const float2* base = lds + gg * RC;
+        float sx = 0.f, sy = 0.f;
+        if (live) {
+#pragma unroll 1
+          for (int j0 = s; j0 < W; j0 += lp) {
+            const float w0 = es_eval(off0 - (float)j0, inv_hw, a.g.beta);
+            float px = 0.f, py = 0.f;
+#pragma unroll
+            for (int j1 = 0; j1 < W; ++j1) {
+              const float2* row = base + ((ls[0] + j0) * R[1] + ls[1] + j1) * R[2] + ls[2];
+              float rx = 0.f, ry = 0.f;
+#pragma unroll
+              for (int j2 = 0; j2 < W; ++j2) {
+                const float2 v = row[j2];
+                rx += wt[2][j2] * v.x;
+                ry += wt[2][j2] * v.y;
+              }
+              px += wt[1][j1] * rx;
+              py += wt[1][j1] * ry;
+            }
+            sx += w0 * px;
+            sy += w0 * py;
+          }
+        }
+        for (int m = 1; m < lp; m <<= 1) {  // lanes of one point are consecutive
+          sx += __shfl_xor(sx, m);
+          sy += __shfl_xor(sy, m);
+        }
+        if (live && s == 0) interp_store(a, set, t0 + gg, p, idx, dens, sx, sy);
+      }
+    }
+    return;
+  }
 
   for (int p = pb + threadIdx.x; p < pe; p += NT) {
     int ls[3];
@@ -1082,7 +1524,7 @@ __global__ __launch_bounds__(NT) void k_interp(InterpArgs a) {
           sx += wt[0][j] * v.x;
           sy += wt[0][j] * v.y;
         }
-      } else if constexpr (D == 2) {
+      } else {
 #pragma unroll
         for (int j0 = 0; j0 < W; ++j0) {
           const float2* row = base + (ls[0] + j0) * R[1] + ls[1];
@@ -1096,41 +1538,8 @@ __global__ __launch_bounds__(NT) void k_interp(InterpArgs a) {
           sx += wt[0][j0] * rx;
           sy += wt[0][j0] * ry;
         }
-      } else {
-        const float off0 = a.g.off[0][p];
-        const float inv_hw = 2.0f / W;
-#pragma unroll 1
-        for (int j0 = 0; j0 < W; ++j0) {
-          const float w0 = es_eval(off0 - (float)j0, inv_hw, a.g.beta);
-          float px = 0.f, py = 0.f;
-#pragma unroll
-          for (int j1 = 0; j1 < W; ++j1) {
-            const float2* row = base + ((ls[0] + j0) * R[1] + ls[1] + j1) * R[2] + ls[2];
-            float rx = 0.f, ry = 0.f;
-#pragma unroll
-            for (int j2 = 0; j2 < W; ++j2) {
-              const float2 v = row[j2];
-              rx += wt[2][j2] * v.x;
-              ry += wt[2][j2] * v.y;
-            }
-            px += wt[1][j1] * rx;
-            py += wt[1][j1] * ry;
-          }
-          sx += w0 * px;
-          sy += w0 * py;
-        }
       }
-      const int64_t ko = ksp_index(a.g, a.ts, set, t0 + gg, idx);
-      float2 r = make_float2(a.scale * sx, a.scale * sy);
-      if (a.mode == 1) {
-        // residual for the spreading that follows: written in sorted order
-        const float2 y = a.obs[ko];
-        r.x = (r.x - y.x) * dens;
-        r.y = (r.y - y.y) * dens;
-        a.out[(int64_t)(t0 + gg) * a.g.M + p] = r;
-      } else {
-        a.out[ko] = r;
-      }
+      interp_store(a, set, t0 + gg, p, idx, dens, sx, sy);
     }
   }
 }
@@ -1314,6 +1723,12 @@ static int region_extent(const hipnufft_plan_s* p, int d) {
   return (d == 2 ? p->interp_span : 1) * p->tile[d] + p->w;
 }
 
+// 16-cell chunks (128-byte lines) a region row along the last axis can meet
+// whatever its alignment
+static int region_chunks(const hipnufft_plan_s* p) {
+  return (region_extent(p, 2) + 15 + 15) / 16;
+}
+
 static void set_region_magic(const hipnufft_plan_s* p, InterpArgs* a, int nt = kThreads) {
   const int o = 3 - p->ndim;
   a->wrap1 = 1;
@@ -1327,6 +1742,11 @@ static void set_region_magic(const hipnufft_plan_s* p, InterpArgs* a, int nt = k
   a->stage_rows = nt / a->stage_lanes;
   a->lane_mag = magic_of(a->stage_lanes);
   a->span = p->interp_span;
+  a->stage_chunks = region_chunks(p);
+  const int64_t cells = grid_size(p);
+  a->mask_stage = p->mask_rows > 0 && p->sub_mask && cells < ((int64_t)1 << 28);
+  a->grid_bytes = a->mask_stage ? (int)(cells * (int64_t)sizeof(float2)) : 0;
+  a->sub_mask = a->mask_stage ? p->sub_mask : nullptr;
 }
 
 // LDS of an interpolation workgroup staging G transforms: the regions and,
@@ -1335,7 +1755,9 @@ static size_t region_lds_bytes(const hipnufft_plan_s* p, int G) {
   size_t cells = 1;
   for (int d = 3 - p->ndim; d < 3; ++d) cells *= (size_t)region_extent(p, d);
   const size_t rows = p->ndim == 1 ? 0 : cells / (size_t)region_extent(p, 2);
-  return sizeof(float2) * cells * (size_t)G + sizeof(int) * rows;
+  // the regions, then per row its offset (unmasked staging) or its table of
+  // chunk offsets (masked staging)
+  return sizeof(float2) * cells * (size_t)G + sizeof(int) * rows * (size_t)region_chunks(p);
 }
 
 // one transform's region may use all of a CU's LDS; several transforms per
@@ -1460,6 +1882,32 @@ int launch_spread_sorted(const hipnufft_plan_s* p, const float2* in, SetRange r,
     return set_error(HIPNUFFT_ERR_ARG,
                      "block spreading needs 3D tiles of 4, 8 or 16 blocks of 4^3 cells, "
                      "at most 4 along an axis");
+  // launches of at most 4 transforms: the matrix kernel with (z cell,
+  // transform, re / im) columns, on blocks of 4 x 4 x (8 / GT) cells
+  int GT = r.ts == 1 ? 1 : r.ts == 2 ? 2 : r.ts <= 4 ? 4 : 0;
+  if (GT) {
+    const int zb = 8 / GT;
+    const int nblk = (p->tile[0] / kBlkE) * (p->tile[1] / kBlkE) * (p->tile[2] / zb);
+    if (p->tile[2] % zb != 0 || nblk > 32) GT = 0;
+  }
+  if (method == HIPNUFFT_SPREAD_MATRIX && GT) {
+    const size_t lds =
+        sizeof(float) * (size_t)kThreads * mfma_record_floats(GT, p->tile[0] + p->tile[1] + p->tile[2]);
+    if (nitems > 0) {
+      const dim3 grd((unsigned)nitems, (unsigned)((r.ts + GT - 1) / GT));
+      StageScope timer(p, kStageSpread);
+      if (GT == 1) hipLaunchKernelGGL(k_spread_mfmaz<1>, grd, dim3(kThreads), lds, p->stream, a);
+      if (GT == 2) hipLaunchKernelGGL(k_spread_mfmaz<2>, grd, dim3(kThreads), lds, p->stream, a);
+      if (GT == 4) hipLaunchKernelGGL(k_spread_mfmaz<4>, grd, dim3(kThreads), lds, p->stream, a);
+      HN_LAUNCH_CHECK();
+    }
+    if (ndense > 0) {
+      const dim3 rg((unsigned)ndense, (unsigned)r.ts);
+      hipLaunchKernelGGL(k_spread_reduce<3>, rg, dim3(kThreads), 0, p->stream, a);
+      HN_LAUNCH_CHECK();
+    }
+    return 0;
+  }
   if (method == HIPNUFFT_SPREAD_MATRIX) {
     // 16 transforms per workgroup (8 waves) when asked for (spread_group >=
     // 16) and a launch has more than 8; measured no faster than 8 at the 3D
@@ -1561,6 +2009,37 @@ int launch_spread_sorted(const hipnufft_plan_s* p, const float2* in, SetRange r,
   return 0;
 }
 
+int build_interp_masks(hipnufft_plan_s* p) {
+  p->mask_rows = 0;
+  if (p->ndim < 2 || p->nsub <= 0 || region_chunks(p) > 16) return 0;
+  int rows = 1;
+  for (int d = 3 - p->ndim; d < 2; ++d) rows *= region_extent(p, d);
+  const int64_t need = p->nsub * (int64_t)rows;
+  if (need > p->sub_mask_cap || !p->sub_mask) {
+    if (p->sub_mask) HN_HIP(hipFree(p->sub_mask));
+    p->sub_mask = nullptr;
+    p->sub_mask_cap = 0;
+    HN_HIP(hipMalloc((void**)&p->sub_mask, sizeof(uint16_t) * need));
+    p->sub_mask_cap = need;
+  }
+  InterpArgs a{};
+  a.g = pts_geom(p);
+  a.sub_tile = p->sub_tile;
+  a.sub_begin = p->sub_begin;
+  a.bin_start = p->bin_start;
+  a.max_sub = p->max_sub;
+  set_region_magic(p, &a);
+  const size_t lds = 2 * sizeof(uint32_t) * (size_t)rows;
+  HN_DIM_SWITCH(p->ndim, HN_WIDTH_SWITCH(p->w, {
+    if constexpr (DD >= 2)
+      hipLaunchKernelGGL((k_interp_masks<DD, WW>), dim3((unsigned)p->nsub), dim3(kThreads), lds,
+                         p->stream, a, p->sub_mask);
+  }));
+  HN_LAUNCH_CHECK();
+  p->mask_rows = rows;
+  return 0;
+}
+
 int launch_interp(const hipnufft_plan_s* p, const float2* grid, int64_t grid_stride, SetRange r,
                   float2* out, const float2* obs, const float* density, float scale, int mode) {
   int rc = check_range(p, r);
@@ -1597,10 +2076,19 @@ int launch_interp(const hipnufft_plan_s* p, const float2* grid, int64_t grid_str
   StageScope timer(p, kStageInterp);
   HN_DIM_SWITCH(p->ndim, HN_WIDTH_SWITCH(p->w, {
     if (nt == kThreads) {
-      hipLaunchKernelGGL((k_interp<DD, WW, kThreads>), grd, dim3(kThreads), lds, p->stream, a);
+      if (a.mask_stage)
+        hipLaunchKernelGGL((k_interp<DD, WW, kThreads, true>), grd, dim3(kThreads), lds,
+                           p->stream, a);
+      else
+        hipLaunchKernelGGL((k_interp<DD, WW, kThreads, false>), grd, dim3(kThreads), lds,
+                           p->stream, a);
+    } else if (a.mask_stage) {
+      if ((rc = allow_lds(k_interp<DD, WW, 2 * kThreads, true>, lds))) return rc;
+      hipLaunchKernelGGL((k_interp<DD, WW, 2 * kThreads, true>), grd, dim3(2 * kThreads), lds,
+                         p->stream, a);
     } else {
-      if ((rc = allow_lds(k_interp<DD, WW, 2 * kThreads>, lds))) return rc;
-      hipLaunchKernelGGL((k_interp<DD, WW, 2 * kThreads>), grd, dim3(2 * kThreads), lds,
+      if ((rc = allow_lds(k_interp<DD, WW, 2 * kThreads, false>, lds))) return rc;
+      hipLaunchKernelGGL((k_interp<DD, WW, 2 * kThreads, false>), grd, dim3(2 * kThreads), lds,
                          p->stream, a);
     }
   }));

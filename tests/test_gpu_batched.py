@@ -146,3 +146,33 @@ def test_dot_rows_bitexact():
         ref = np.sum(ref * b.cpu().numpy().astype(np.complex128), axis=1)
         np.testing.assert_allclose(rows[:, 0].cpu().numpy() + 1j * rows[:, 1].cpu().numpy(), ref,
                                    rtol=1e-12, atol=1e-9)
+
+
+@pytest.mark.parametrize("dim,sense", [(2, True), (2, False), (3, True)])
+def test_multi_trajectory_vs_oracle(dim, sense):
+    """Each problem of the multi-trajectory operator against the exact FP64
+    operator of the oracle on its own trajectory (op, adj_op with density,
+    data consistency), not only against a second hipnufft operator."""
+    from hipnufft.batched import MultiTrajectoryOperator
+    from oracle import mri
+
+    trajs = ([t for t in _trajs_2d() if len(t)] if dim == 2 else _trajs_3d())
+    shape = (40, 36) if dim == 2 else (16, 20, 12)
+    C, K = 3, len(trajs)
+    smaps = gauss_smaps(C, shape, 41) if sense else None
+    dens = [np.random.default_rng(42 + k).uniform(0.5, 2, len(t)).astype(np.float32)
+            for k, t in enumerate(trajs)]
+    op = MultiTrajectoryOperator(trajs, shape, n_coils=C, smaps=smaps, density=dens)
+    x = rand_cplx(op.img_full_shape, 43)
+    ys = [rand_cplx((C, len(t)), 44 + k) for k, t in enumerate(trajs)]
+    got_y = op.split(op.op(torch.from_numpy(x).to(DEV)))
+    got_x = op.adj_op(ys).cpu().numpy()
+    got_dc = op.data_consistency(torch.from_numpy(x).to(DEV), ys).cpu().numpy()
+    for k, t in enumerate(trajs):
+        om = (t * np.float32(2 * np.pi)).astype(np.float32)
+        ref = mri.Operator(om, shape, n_coils=C, smaps=smaps, density=dens[k], exact=True)
+        xk = x[k]
+        assert rel_l2(got_y[k].cpu().numpy(), ref.op(xk.reshape(1, -1, *shape))[0]) < 2e-5, k
+        assert rel_l2(got_x[k], ref.adj_op(ys[k].reshape(1, C, -1))[0]) < 2e-5, k
+        assert rel_l2(got_dc[k], ref.data_consistency(xk.reshape(1, -1, *shape),
+                                                      ys[k].reshape(1, C, -1))[0]) < 2e-5, k
