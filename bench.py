@@ -522,6 +522,18 @@ def main():
     cg = {"p50": float(np.percentile(lat_np, 50)) if lat else None,
           "p99": float(np.percentile(lat_np, 99)) if lat else None,
           "runs": len(lat), "iterations": args.cg_iter, "graph": True, "first_ms": first}
+    if args.cg_runs > 0:
+        # the step size every CG run estimates first: 10 power iterations with the
+        # reference's stop rule (|norm_k - norm_k-1| < 1e-6, absolute), which float32
+        # norms of this size rarely meet, so the estimate runs all 10 iterations
+        import warnings
+
+        with warnings.catch_warnings(record=True) as caught:
+            warnings.simplefilter("always")
+            L = op.get_lipschitz_cst(seed=0)
+        cg["lipschitz"] = {"value": L, "power_iterations": 10,
+                           "stopped_early": not any("converge" in str(w.message)
+                                                    for w in caught)}
     log(f"cg latency {cg}")
 
     # the same CG with Toeplitz gradients (FFT-only iterations); the untimed

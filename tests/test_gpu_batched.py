@@ -169,8 +169,9 @@ def test_multi_trajectory_vs_oracle(dim, sense):
     got_x = op.adj_op(ys).cpu().numpy()
     got_dc = op.data_consistency(torch.from_numpy(x).to(DEV), ys).cpu().numpy()
     for k, t in enumerate(trajs):
-        om = (t * np.float32(2 * np.pi)).astype(np.float32)
-        ref = mri.Operator(om, shape, n_coils=C, smaps=smaps, density=dens[k], exact=True)
+        # unit samples: the oracle rescales them to radians as the operator does
+        # (radians of a trajectory inside [-0.08, 0.08) would be rescaled twice)
+        ref = mri.Operator(t, shape, n_coils=C, smaps=smaps, density=dens[k], exact=True)
         xk = x[k]
         assert rel_l2(got_y[k].cpu().numpy(), ref.op(xk.reshape(1, -1, *shape))[0]) < 2e-5, k
         assert rel_l2(got_x[k], ref.adj_op(ys[k].reshape(1, C, -1))[0]) < 2e-5, k
