@@ -93,6 +93,8 @@ struct InterpArgs {
   int mask_stage;    // stage only the chunks the points read (needs grid_bytes)
   int grid_bytes;    // bytes of one transform's grid, < 2^31
   const uint16_t* sub_mask;  // per work item and region row: the chunks read
+  int lds_pad;               // k_interp_stream: LDS cells after each region row
+  uint32_t lds_row_mag;      // and the magic multiplier of the padded row length
 };
 
 // dword 3 of a raw buffer resource on gfx9-family GPUs (gfx950 included):
@@ -1544,6 +1546,205 @@ __global__ __launch_bounds__(NT, 4) void k_interp(InterpArgs a) {
   }
 }
 
+// k_interp_stream configuration: threads per workgroup and LDS buffers (512
+// threads with one buffer, two workgroups per CU, ran the 16-coil 192^3
+// interpolation in 5.5 ms; 1024 threads with two buffers in 6.6 ms; 512
+// with two, one workgroup per CU, 7.1 ms), region cells a buffer holds
+// (8 x 8 x 16 tiles, two along the last axis, w <= 8), region rows the row
+// tables hold, the widest kernel it is built for (its footprint loops unroll
+// fully) and the LDS cells after each region row (rows of 41 or 42 cells
+// measured the same as 39, rows of 40 slower)
+constexpr int kStreamNT = 512;
+constexpr int kStreamNB = 1;
+constexpr int kStreamCells = 9728;
+constexpr int kStreamRows = 256;
+constexpr int kStreamMaxW = 8;
+constexpr int kStreamPad = 0;
+
+// 3D interpolation of many transforms of a work item in one workgroup.
+// What does not depend on the transform is done once per item: the grid
+// byte offset of every staged 32-bit word of the region (each thread owns
+// the words tid + u * NT, u < K; a word of a chunk no point reads gets an
+// offset past the grid's end, which loads nothing), and each point's kernel
+// weights.  Each transform's region goes HBM -> LDS by direct-to-LDS buffer
+// loads (no registers hold it; a wave's load writes 64 consecutive words,
+// which the flat word order of the region gives), then its points gather
+// their footprints from LDS.  With one buffer (NB = 1) two workgroups share
+// a CU and one's loads overlap the other's gathers; with two (NB = 2) the
+// loads of transform gg + 1 are issued before transform gg is gathered.
+// Points are spread over lp = 1, 2, 4 or 8 consecutive lanes (as many as
+// the item allows; it holds at most NT >> LGMIN points), lane s summing
+// the kernel rows j0 = s, s + lp, ...; the partial sums meet by lane
+// exchanges in a fixed order, so the result does not depend on the run.
+// The footprint gathers bound the kernel: random 16-byte LDS reads.
+template <int W, int NT, int NB>
+__global__ __launch_bounds__(NT, 4) void k_interp_stream(InterpArgs a) {
+  // static LDS (NB buffers and the row tables): the compiler then knows how
+  // many workgroups share a CU
+  __shared__ float2 lds[NB * kStreamCells + kStreamRows];
+  constexpr int K = 2 * kStreamCells / NT;  // staged words per thread
+  constexpr int D = 3;
+  constexpr uint32_t kSkip = 0x80000000u;
+  constexpr int LGMIN = NT >= 1024 ? 1 : 0;  // items hold at most NT >> LGMIN points
+  constexpr int KJ = (W + (1 << LGMIN) - 1) >> LGMIN;  // rows per lane at the fewest lanes
+  const int sub = a.sub0 + (int)xcd_remap(blockIdx.x, gridDim.x);
+  const int key = a.sub_tile[sub];
+  const int set = key / a.g.nbins;
+  const int tile = key - set * a.g.nbins;
+  const int pb = a.sub_begin[sub];
+  const int pe = min(pb + a.max_sub, a.bin_start[key + span_bins<D>(a, tile)]);
+  const int t0 = blockIdx.y * a.G;
+  const int G = min(a.G, a.ts - t0);
+  const int tb = (set - a.k0) * a.ts + t0;  // chunk transform of the first handled here
+  int rorg[3], R[3];
+  region_of<D, W>(a, tile, rorg, R);
+  const int NR = R[0] * R[1];
+  const int RL = R[2] + a.lds_pad;  // LDS row length: the padding moves rows across banks
+  const int RC = NR * RL;
+  const int RCP = (RC + 31) & ~31;  // a buffer: whole 64-word wave segments
+  const int tid = (int)threadIdx.x;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+
+  // per region row: wrapped grid offset of its first cell, chunk mask
+  int* rowoff = reinterpret_cast<int*>(lds + NB * RCP);
+  uint32_t* rowmask = reinterpret_cast<uint32_t*>(rowoff + NR);
+  for (int r = tid; r < NR; r += NT) {
+    const int l0 = mdiv(r, a.rmag[1]);
+    const int l1 = r - l0 * R[1];
+    rowoff[r] = (wrap_coord(rorg[0] + l0, a.g.nf[0], a.wrap1) * a.g.nf[1] +
+                 wrap_coord(rorg[1] + l1, a.g.nf[1], a.wrap1)) * a.g.nf[2];
+    rowmask[r] = a.sub_mask ? (uint32_t)a.sub_mask[(int64_t)sub * NR + r] : 0xffffffffu;
+  }
+  __syncthreads();
+  const int phase = ((rorg[2] % 16) + 16) % 16;
+  uint32_t off[K];
+#pragma unroll
+  for (int u = 0; u < K; ++u) {
+    const int wd = tid + u * NT;  // word: cell wd / 2, real (0) or imaginary (1) part
+    const int c = wd >> 1;
+    off[u] = kSkip;
+    if (c < RC) {
+      const int r = mdiv(c, a.lds_row_mag);
+      const int z = c - r * RL;
+      if (z < R[2] && ((rowmask[r] >> ((z + phase) >> 4)) & 1u))
+        off[u] = (uint32_t)(rowoff[r] + wrap_coord(rorg[2] + z, a.g.nf[2], a.wrap1)) * 8u +
+                 4u * (uint32_t)(wd & 1);
+    }
+  }
+
+  // the points: one trip of lp lanes each
+  const int n = pe - pb;
+  int lg = LGMIN;
+  while (lg < 3 && (n << (lg + 1)) <= NT) ++lg;
+  const int lp = 1 << lg;
+  const int p = pb + (tid >> lg);
+  const int s = tid & (lp - 1);
+  const bool live = p < pe;
+  int lbase = 0;  // the lane's first footprint row j0 = s, as an LDS cell index
+  float wt1[W], wt2[W], w0[KJ];
+  int64_t kbase = 0, kstep = 0;  // k-space offset of transform t: kbase + t * kstep
+  float dens = 1.0f;
+#pragma unroll
+  for (int j = 0; j < W; ++j) wt1[j] = wt2[j] = 0.0f;
+#pragma unroll
+  for (int k = 0; k < KJ; ++k) w0[k] = 0.0f;
+  if (live) {
+    int ls[3];
+    float wt[D][W];
+    point_setup<D, W>(a, p, rorg, ls, wt);
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+      wt1[j] = wt[1][j];
+      wt2[j] = wt[2][j];
+    }
+    const float inv_hw = 2.0f / W;
+    const float off0 = a.g.off[0][p];
+#pragma unroll
+    for (int k = 0; k < KJ; ++k)  // the weights of the lane's rows s + k lp
+      w0[k] = s + k * lp < W ? es_eval(off0 - (float)(s + k * lp), inv_hw, a.g.beta) : 0.0f;
+    lbase = ((ls[0] + s) * R[1] + ls[1]) * RL + ls[2];
+    const int idx = a.g.idx[p];
+    dens = (a.mode == 1 && a.density) ? a.density[idx] : 1.0f;
+    kbase = ksp_index(a.g, a.ts, set, 0, idx);
+    kstep = ksp_index(a.g, a.ts, set, 1, idx) - kbase;
+  }
+  const int rstep = lp * R[1] * RL;  // LDS cells between the lane's rows
+
+  // the region of transform t into buffer b: load u of wave w fills words
+  // [u NT + 64 w, u NT + 64 w + 64) of the buffer
+  auto stage = [&](int t, int b) {
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)(a.grid + (int64_t)(tb + t) * a.grid_stride), (short)0, a.grid_bytes,
+        kBufferWord3);
+    float* buf = reinterpret_cast<float*>(lds + b * RCP);
+#pragma unroll
+    for (int u = 0; u < K; ++u) {
+      const int w0d = u * NT + wave * 64;  // uniform
+      if (w0d < 2 * RCP)
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(
+            rs, (__attribute__((address_space(3))) void*)(buf + w0d), 4, off[u], 0, 0, 0);
+    }
+  };
+  if (NB == 2) stage(0, 0);
+  for (int gg = 0; gg < G; ++gg) {
+    float2 y = make_float2(0.f, 0.f);
+    if constexpr (NB == 2) {
+      // this thread's loads of transform gg have landed; after the barrier
+      // every thread's have, and every lane is done with transform gg - 1
+      __builtin_amdgcn_s_waitcnt(0);
+      __syncthreads();
+      // the residual's observation before the next loads, so that waiting
+      // for it leaves them in flight
+      if (a.mode == 1 && live && s == 0) y = a.obs[kbase + (t0 + gg) * kstep];
+      if (gg + 1 < G) stage(gg + 1, (gg + 1) & 1);
+    } else {
+      if (gg > 0) __syncthreads();  // every lane is done with transform gg - 1
+      if (a.mode == 1 && live && s == 0) y = a.obs[kbase + (t0 + gg) * kstep];
+      stage(gg, 0);
+      __builtin_amdgcn_s_waitcnt(0);
+      __syncthreads();
+    }
+    const float2* cur = lds + (NB == 2 ? (gg & 1) * RCP : 0);
+    float sx = 0.f, sy = 0.f;
+#pragma unroll
+    for (int k = 0; k < KJ; ++k) {
+      if (live && s + k * lp < W) {
+        const float2* rowk = cur + lbase + k * rstep;
+        float px = 0.f, py = 0.f;
+#pragma unroll
+        for (int j1 = 0; j1 < W; ++j1) {
+          const float2* row = rowk + j1 * RL;
+          float rx = 0.f, ry = 0.f;
+#pragma unroll
+          for (int j2 = 0; j2 < W; ++j2) {
+            const float2 c = row[j2];
+            rx += wt2[j2] * c.x;
+            ry += wt2[j2] * c.y;
+          }
+          px += wt1[j1] * rx;
+          py += wt1[j1] * ry;
+        }
+        sx += w0[k] * px;
+        sy += w0[k] * py;
+      }
+    }
+    for (int m = 1; m < lp; m <<= 1) {
+      sx += __shfl_xor(sx, m);
+      sy += __shfl_xor(sy, m);
+    }
+    if (live && s == 0) {
+      float2 r = make_float2(a.scale * sx, a.scale * sy);
+      if (a.mode == 1) {
+        r.x = (r.x - y.x) * dens;
+        r.y = (r.y - y.y) * dens;
+        a.out[(int64_t)(t0 + gg) * a.g.M + p] = r;
+      } else {
+        a.out[kbase + (t0 + gg) * kstep] = r;
+      }
+    }
+  }
+}
+
 // gtraj[idx*D + d] += Re sum_t conj(dy[t][idx]) * scale * d/dx_d interp_t
 template <int D, int W>
 __global__ __launch_bounds__(kThreads) void k_grad_traj(InterpArgs a) {
@@ -2064,7 +2265,38 @@ int launch_interp(const hipnufft_plan_s* p, const float2* grid, int64_t grid_str
   a.mode = mode;
   if (region_lds_bytes(p, 1) > kMaxLdsRegion)
     return set_error(HIPNUFFT_ERR_ARG, "interp tile too large for LDS");
-  int G = std::min(p->interp_group, r.ts);
+  if (p->ndim == 3 && p->interp_group <= 0) {
+    // every transform of an item in one workgroup (k_interp_stream) when
+    // its region, grid and items fit the kernel
+    constexpr int NT = kStreamNT, NB = kStreamNB;
+    int64_t cells = 1;
+    for (int d = 0; d < 3; ++d) cells *= region_extent(p, d);
+    const int64_t rows = cells / region_extent(p, 2);
+    const int64_t padded = (rows * (region_extent(p, 2) + kStreamPad) + 31) / 32 * 32;
+    if (padded <= kStreamCells && rows <= kStreamRows && grid_size(p) < ((int64_t)1 << 28) &&
+        p->max_sub <= (NT >= 1024 ? NT / 2 : NT) && p->w <= kStreamMaxW) {
+      set_region_magic(p, &a, NT);
+      a.lds_pad = kStreamPad;
+      a.lds_row_mag = magic_of(region_extent(p, 2) + kStreamPad);
+      a.grid_bytes = (int)(grid_size(p) * (int64_t)sizeof(float2));
+      a.sub_mask = p->mask_rows > 0 ? p->sub_mask : nullptr;
+      // transforms per workgroup: all of the set's, fewer when the items
+      // alone would not fill the GPU several times over
+      int Gs = r.ts;
+      while (Gs > 1 && nsub * (int64_t)((r.ts + Gs - 1) / Gs) < 1024) Gs = (Gs + 1) / 2;
+      a.G = Gs;
+      const dim3 grd((unsigned)nsub, (unsigned)((r.ts + Gs - 1) / Gs));
+      StageScope timer(p, kStageInterp);
+      HN_WIDTH_SWITCH(p->w, {
+        if constexpr (WW <= kStreamMaxW) {
+          hipLaunchKernelGGL((k_interp_stream<WW, NT, NB>), grd, dim3(NT), 0, p->stream, a);
+        }
+      });
+      HN_LAUNCH_CHECK();
+      return 0;
+    }
+  }
+  int G = std::max(1, std::min(p->interp_group, r.ts));
   while (G > 1 && region_lds_bytes(p, G) > kMaxLds) --G;
   a.G = G;
   const size_t lds = region_lds_bytes(p, G);

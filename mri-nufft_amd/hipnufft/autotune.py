@@ -148,8 +148,10 @@ def autotune(coords, shape, n_trans=1, eps=1e-6, upsampfac=2.0, reps=3, budget_s
         ("tile", tile_candidates(ndim, w)),
         ("spread_method", [1, 2, 4] if ndim == 3 else []),
         ("spread_group", [4, 8, 16]),
+        # 3D: 0 interpolates every transform of a work item in one workgroup
+        # (items of at most 512 points), 1 one transform per workgroup
         ("interp_group", [1, 2, 4] if ndim < 3 else [1]),
-        ("max_subproblem", [512, 1024, 2048]),
+        ("max_subproblem", [512, 1024, 2048] if ndim < 3 else [64, 256, 1024]),
         ("spread_max", [256, 512, 1024, 2048, 4096]),
     ]
     for name, values in stages:

@@ -194,6 +194,47 @@ def test_spread_kernels_bit_identical(T):
         assert rel_l2(outs["blocks"][t].cpu().numpy(), exp) < 1e-5
 
 
+INTERP_KERNELS = [
+    ("stream", {}),                                   # every transform of an item per workgroup
+    ("stream-small-items", {"max_subproblem": 40}),   # lp = 4 lanes per point, many items
+    ("stream-span1", {"interp_span": 1}),
+    ("stream-span3", {"interp_span": 3}),
+    ("per-transform", {"interp_group": 1}),           # one transform per workgroup
+    ("fallback-big-items", {"max_subproblem": 700}),  # items too large for the stream kernel
+]
+
+
+@pytest.mark.parametrize("T", [1, 3, 16, 19])
+@pytest.mark.parametrize("shape", [(24, 20, 32), (6, 4, 5)], ids=["grid", "tiny-wrapping"])
+def test_interp_kernels_3d(shape, T):
+    """3D type 2 through both interpolation kernels (all transforms of a work
+    item in one workgroup, staged by direct-to-LDS loads; one transform per
+    workgroup) and their work-item splits: each transform against the FP64
+    oracle NUFFT (<= 1e-5), and the kernels against each other (<= 1e-6).
+    The tiny grid is smaller than the kernel footprint, so regions wrap
+    several times around it."""
+    from hipnufft import _capi
+
+    traj = np.concatenate([rand_traj(3000, 3, 51), rand_traj(2500, 3, 52) * 0.06])
+    om = torch.from_numpy((traj * np.float32(2 * np.pi)).astype(np.float32)).to(DEV)
+    x = torch.from_numpy(rand_cplx((T, *shape), seed=53)).to(DEV)
+    outs = {}
+    for name, kw in INTERP_KERNELS:
+        p = _capi.RawPlan(shape, n_trans=T, **kw)
+        p.setpts(om)
+        y = torch.empty((T, traj.shape[0]), dtype=torch.complex64, device=DEV)
+        p.execute(x, y)
+        outs[name] = y.cpu().numpy()
+        p.close()
+    ref = nufft.Plan(om.cpu().numpy(), shape, eps=1e-6)
+    for t in sorted({0, T - 1}):
+        exp = ref.execute(x[t].cpu().numpy().astype(np.complex128))
+        for name in outs:
+            assert rel_l2(outs[name][t], exp) < 1e-5, (name, t)
+    for name in outs:
+        assert rel_l2(outs[name], outs["per-transform"]) < 1e-6, name
+
+
 @pytest.mark.parametrize("T", [5, 12])
 @pytest.mark.parametrize("tile,shape,eps", [
     ((8, 8, 8), (16, 12, 10), 1e-6),     # fine grid 32 x 24 x 20: partial edge tiles
