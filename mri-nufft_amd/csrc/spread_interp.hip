@@ -302,6 +302,7 @@ __global__ __launch_bounds__(kThreads) void k_spread(SpreadArgs a) {
 // the same bits.
 constexpr int kBlkE = 4;            // block edge
 constexpr int kBlkPerWave = 4;      // at most 16 blocks per tile
+constexpr int kSpreadWaves = 4;     // k_spread_mfma waves per 8 transforms (4 or 8)
 constexpr int kBlkChunk = 128;      // staged entries per round
 constexpr int kBlkPad = 3;          // zeros before / after each weight row
 constexpr int kBlkHead = 4;         // mask + one word of 4 byte offsets per axis
@@ -589,17 +590,16 @@ __global__ __launch_bounds__(kThreads) void k_spread_blocks(SpreadArgs a) {
 // D[4 (l / 16) + r][l % 16] in register r.  With GH = 2 the A operand (and
 // the staging and the entry selection behind it) serves both halves of 16
 // transforms.  A wave takes the entries touching each of its blocks four at
-// a time in list order (a ballot per 64 staged entries, windows aligned the
-// same way for every GH), so every cell's sum has the same order on every
-// run and for both GH: deterministic, equal to the other kernels to
-// rounding.
+// a time in list order (after each staging round a wave lists, per block,
+// the staged entries touching it: a ballot and a lane prefix count per 64
+// entries), so every cell's sum has the same order on every run and for
+// both GH: deterministic, equal to the other kernels to rounding.
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-template <int GH>
-__global__ __launch_bounds__(4 * 64 * GH, 4 / GH) void k_spread_mfma(SpreadArgs a) {
+template <int GH, int NWAVE>
+__global__ __launch_bounds__(64 * NWAVE, 4 / GH) void k_spread_mfma(SpreadArgs a) {
   constexpr int G = 8 * GH;              // transforms per workgroup
-  constexpr int NWAVE = 4 * GH;          // waves per workgroup
-  constexpr int BPW = kBlkPerWave / GH;  // blocks per wave (16 blocks at most)
+  constexpr int BPW = 16 / NWAVE;        // blocks per wave (16 blocks at most)
   extern __shared__ float smem[];
   const int PW = mfma_record_floats(G, a.g.tile[0] + a.g.tile[1] + a.g.tile[2]);
   const int item = a.item0 + blockIdx.x;
@@ -649,10 +649,25 @@ __global__ __launch_bounds__(4 * 64 * GH, 4 / GH) void k_spread_mfma(SpreadArgs 
 #pragma unroll
       for (int h = 0; h < GH; ++h) acc[bi][s][h] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-  constexpr int CHUNK = kBlkChunk * GH;  // staged entries per round (a multiple of 64)
+  constexpr int CHUNK = kBlkChunk * NWAVE / 4;  // staged entries per round (a multiple of 64)
   const int e0 = a.it_begin[item];
   const int e1 = min(e0 + a.smax, a.list_start[key + 1]);
   const int q = (int)threadIdx.x;
+  // per block of the wave, the record offsets (floats from smem) of the
+  // staged entries that touch it, in list order; a zero record after the
+  // staged ones pads the last group of four
+  const int zrec = CHUNK * PW;
+  int16_t* lists = reinterpret_cast<int16_t*>(smem + zrec + PW) + wave * BPW * (CHUNK + 4);
+  for (int i = (int)threadIdx.x; i < PW; i += 64 * NWAVE) smem[zrec + i] = 0.0f;
+  // a lane's operands sit at fixed offsets in a record (block weights, value)
+  int ow4[BPW], owy[BPW], owz[BPW];
+#pragma unroll
+  for (int bi = 0; bi < BPW; ++bi) {
+    ow4[bi] = kBlkHead + 2 * G + kBlkE * bidx[bi][0];
+    owy[bi] = kBlkHead + 2 * G + td[0] + kBlkE * bidx[bi][1] + cy;
+    owz[bi] = kBlkHead + 2 * G + td[0] + td[1] + kBlkE * bidx[bi][2] + cz;
+  }
+  const int ov = kBlkHead + (lane & 15);
   for (int eb = e0; eb < e1; eb += CHUNK) {
     const int n = min(CHUNK, e1 - eb);
     if (q < n) {
@@ -661,45 +676,53 @@ __global__ __launch_bounds__(4 * 64 * GH, 4 / GH) void k_spread_mfma(SpreadArgs 
       write_mfma_entry<G>(smem + q * PW, r, a, t0, Gv, td, ext, nf, nb, inv_hw, beta);
     }
     __syncthreads();
-    for (int base = 0; base < n; base += 64) {
-      const int qi = base + lane;
-      const uint32_t hm = qi < n ? reinterpret_cast<const uint32_t*>(smem + qi * PW)[0] : 0u;
+    int ngroups[BPW];
 #pragma unroll
-      for (int bi = 0; bi < BPW; ++bi) {
-        if (!bbit[bi]) continue;  // uniform
-        uint64_t todo = __ballot((hm & bbit[bi]) != 0u);
-        while (todo) {  // uniform: four entries per step, list order
-          int ek[4];
-#pragma unroll
-          for (int k = 0; k < 4; ++k) {
-            ek[k] = todo ? __builtin_ctzll(todo) : -1;
-            todo &= todo - 1;
-          }
-          const int e = kq == 0 ? ek[0] : kq == 1 ? ek[1] : kq == 2 ? ek[2] : ek[3];
-          float av[kBlkE] = {0.f, 0.f, 0.f, 0.f};
-          float bv[GH];
-#pragma unroll
-          for (int h = 0; h < GH; ++h) bv[h] = 0.f;
-          if (e >= 0) {
-            const float* sp = smem + (base + e) * PW;
-            const float* wx = sp + kBlkHead + 2 * G;
-            const float4 w4 = *reinterpret_cast<const float4*>(wx + kBlkE * bidx[bi][0]);
-            const float wyz = wx[td[0] + kBlkE * bidx[bi][1] + cy] *
-                              wx[td[0] + td[1] + kBlkE * bidx[bi][2] + cz];
-            av[0] = w4.x * wyz;
-            av[1] = w4.y * wyz;
-            av[2] = w4.z * wyz;
-            av[3] = w4.w * wyz;
-#pragma unroll
-            for (int h = 0; h < GH; ++h) bv[h] = sp[kBlkHead + 16 * h + (lane & 15)];
-          }
-#pragma unroll
-          for (int sx = 0; sx < kBlkE; ++sx)
-#pragma unroll
-            for (int h = 0; h < GH; ++h)
-              acc[bi][sx][h] =
-                  __builtin_amdgcn_mfma_f32_16x16x4f32(av[sx], bv[h], acc[bi][sx][h], 0, 0, 0);
+    for (int bi = 0; bi < BPW; ++bi) {
+      int16_t* lst = lists + bi * (CHUNK + 4);
+      int cnt = 0;
+      if (bbit[bi]) {  // uniform
+        for (int base = 0; base < n; base += 64) {
+          const int qi = base + lane;
+          const uint32_t hm =
+              qi < n ? reinterpret_cast<const uint32_t*>(smem + qi * PW)[0] : 0u;
+          const bool hit = (hm & bbit[bi]) != 0u;
+          const uint64_t bal = __ballot(hit);
+          if (hit) lst[cnt + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
+                                   __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u))] =
+              (int16_t)(qi * PW);
+          cnt += __popcll(bal);
         }
+        if (lane < 4) lst[cnt + lane] = (int16_t)zrec;
+      }
+      ngroups[bi] = (cnt + 3) >> 2;
+    }
+#pragma unroll
+    for (int bi = 0; bi < BPW; ++bi) {
+      const int16_t* lst = lists + bi * (CHUNK + 4);
+      // four entries per step (uniform count); the next step's record offset
+      // is read before this step's products
+      int o = ngroups[bi] > 0 ? lst[kq] : 0;
+      for (int gq = 0; gq < ngroups[bi]; ++gq) {
+        const int on = lst[4 * gq + 4 + kq];  // <= CHUNK + 3: inside the list
+        const float* sp = smem + o;
+        const float4 w4 = *reinterpret_cast<const float4*>(sp + ow4[bi]);
+        const float wyz = sp[owy[bi]] * sp[owz[bi]];
+        float av[kBlkE];
+        av[0] = w4.x * wyz;
+        av[1] = w4.y * wyz;
+        av[2] = w4.z * wyz;
+        av[3] = w4.w * wyz;
+        float bv[GH];
+#pragma unroll
+        for (int h = 0; h < GH; ++h) bv[h] = sp[ov + 16 * h];
+#pragma unroll
+        for (int sx = 0; sx < kBlkE; ++sx)
+#pragma unroll
+          for (int h = 0; h < GH; ++h)
+            acc[bi][sx][h] =
+                __builtin_amdgcn_mfma_f32_16x16x4f32(av[sx], bv[h], acc[bi][sx][h], 0, 0, 0);
+        o = on;
       }
     }
     __syncthreads();
@@ -2114,17 +2137,23 @@ int launch_spread_sorted(const hipnufft_plan_s* p, const float2* in, SetRange r,
     // 16) and a launch has more than 8; measured no faster than 8 at the 3D
     // headline, so 8 by default
     const int GH = (r.ts > 8 && p->spread_group >= 16) ? 2 : 1;
+    const int NW = GH == 2 ? 8 : kSpreadWaves;
     a.G = 8 * GH;
-    const size_t lds = sizeof(float) * (size_t)kBlkChunk * GH *
-                       mfma_record_floats(8 * GH, p->tile[0] + p->tile[1] + p->tile[2]);
+    const size_t chunk = (size_t)kBlkChunk * NW / 4;
+    const size_t lds =
+        sizeof(float) * (chunk + 1) *  // the records and a zero record
+            mfma_record_floats(8 * GH, p->tile[0] + p->tile[1] + p->tile[2]) +
+        sizeof(int16_t) * 16 * (chunk + 4);  // the per-block entry lists
     if (nitems > 0) {
       const dim3 grd((unsigned)nitems, (unsigned)((r.ts + 8 * GH - 1) / (8 * GH)));
       StageScope timer(p, kStageSpread);
       if (GH == 2) {
-        if ((rc = allow_lds(k_spread_mfma<2>, lds))) return rc;
-        hipLaunchKernelGGL(k_spread_mfma<2>, grd, dim3(2 * kThreads), lds, p->stream, a);
+        if ((rc = allow_lds(k_spread_mfma<2, 8>, lds))) return rc;
+        hipLaunchKernelGGL((k_spread_mfma<2, 8>), grd, dim3(2 * kThreads), lds, p->stream, a);
       } else {
-        hipLaunchKernelGGL(k_spread_mfma<1>, grd, dim3(kThreads), lds, p->stream, a);
+        if ((rc = allow_lds(k_spread_mfma<1, kSpreadWaves>, lds))) return rc;
+        hipLaunchKernelGGL((k_spread_mfma<1, kSpreadWaves>), grd, dim3(64 * kSpreadWaves), lds,
+                           p->stream, a);
       }
       HN_LAUNCH_CHECK();
     }
