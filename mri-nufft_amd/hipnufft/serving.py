@@ -11,11 +11,12 @@ and workspaces stay resident in an LRU cache keyed by the trajectory, so a
 repeated protocol pays for its sort once; coil maps are uploaded per request
 into the resident plan.
 
-Every CG iteration after the first two is a replay of a HIP graph captured
-per slot, and the stopping test is a device flag (as in
-``pinv_solver(optim="cg", graph=True)``), so a request gives the same result
-alone and among others: same kernels, same summation order, and a Lipschitz
-constant computed from a seeded start vector.  By default each resident plan
+The stopping test of every CG iteration is a device flag (as in
+``pinv_solver(optim="cg", graph=True)``), so no iteration waits for the host
+and a request gives the same result alone and among others: same kernels,
+same summation order, and a Lipschitz constant computed from a seeded start
+vector.  With ``use_graph=True`` every iteration after the first two is a
+replay of a HIP graph captured per request.  By default each resident plan
 also holds the Toeplitz form of its Gram operator, so the CG gradients
 ``A^H (A x - y)`` are evaluated as ``G x - A^H y`` with FFTs only (``A^H y``
 once per request); ``toeplitz=False`` uses the fused NUFFT data consistency.
@@ -360,7 +361,7 @@ class _CGJob:
 class ReconServer:
     """Slot-based continuous batching under an HBM budget (see module doc)."""
 
-    def __init__(self, hbm_budget_bytes=64 << 30, max_slots=8, use_graph=True, seed=1234,
+    def __init__(self, hbm_budget_bytes=64 << 30, max_slots=8, use_graph=False, seed=1234,
                  toeplitz=True, depth=1, stacked=True):
         if not MRIHipNUFFT.available:
             raise RuntimeError("hipnufft is not available (library or GPU missing)")
@@ -373,6 +374,11 @@ class ReconServer:
         self.depth = int(depth)
         self.budget = int(hbm_budget_bytes)
         self.max_slots = int(max_slots)
+        # capture each request's CG iteration in a HIP graph and replay it:
+        # off by default, the mixed 10-iteration trace measured 45-54 req/s
+        # with graphs off against 44-52 with them on a new server, and
+        # 122-135 against 108-130 with resident plans (a capture per request
+        # costs more than its replays save)
         self.use_graph = bool(use_graph)
         self.seed = int(seed)
         self.streams = [torch.cuda.Stream() for _ in range(self.max_slots)]

@@ -31,12 +31,12 @@ def _requests():
     return reqs
 
 
-@pytest.mark.parametrize("toeplitz", [True, False])
-def test_alone_vs_mix_identical(toeplitz):
+@pytest.mark.parametrize("toeplitz,graph", [(True, False), (False, False), (True, True)])
+def test_alone_vs_mix_identical(toeplitz, graph):
     from hipnufft.serving import ReconServer
 
     reqs = _requests()
-    mix = ReconServer(max_slots=4, toeplitz=toeplitz).run(reqs, realtime=False)
+    mix = ReconServer(max_slots=4, toeplitz=toeplitz, use_graph=graph).run(reqs, realtime=False)
     assert sorted(mix) == list(range(len(reqs)))
     # all requests wait from the start: latencies count from the call
     assert all(m.arrival == 0.0 and 0.0 <= m.admitted <= m.finished for m in mix.values())
@@ -45,7 +45,7 @@ def test_alone_vs_mix_identical(toeplitz):
     for r in reqs:
         assert rel_l2(deep[r.rid].image, mix[r.rid].image) <= 1e-6, r.rid
         alone = ReconServer(max_slots=1, toeplitz=toeplitz).run([r], realtime=False)[r.rid]
-        assert rel_l2(mix[r.rid].image, alone.image) <= 1e-6, r.rid
+        assert rel_l2(mix[r.rid].image, alone.image) <= 1e-6, r.rid  # graph replays == eager
 
 
 @pytest.mark.parametrize("toeplitz,tol", [(False, 1e-5), (True, 1e-4)])
