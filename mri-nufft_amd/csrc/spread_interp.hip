@@ -387,6 +387,48 @@ __device__ __forceinline__ void write_blk_entry(float* sp, const BlkPos& r, cons
   }
 }
 
+// Kernel weights of one axis of a matrix-kernel record: wr[x] for the td
+// tile cells x (zero outside the footprint and past the grid end); returns
+// the bits of the blocks of `edge` cells along the axis that the footprint
+// meets.  When nf >= td + W the footprint's copies [st + k nf, st + k nf + W)
+// lie nf apart, so one of them meets the tile: it starts at tile cell s, the
+// representative of st mod nf in (-W, td), and only its W cells are
+// evaluated.  Smaller grids take the cell-by-cell form.  A4: wr is 16-byte
+// aligned (the zero fill is written four cells at a time).
+template <bool A4>
+__device__ __forceinline__ uint32_t axis_weights(float* wr, int st, float off, int nf, int td,
+                                                 int ext, int W, int edge, float inv_hw,
+                                                 float beta) {
+  uint32_t hit = 0u;
+  if (nf >= td + W) {
+    int s = st < 0 ? st + nf : st;
+    if (s >= td) s -= nf;
+    if constexpr (A4) {
+      for (int x = 0; x < td; x += 4)
+        *reinterpret_cast<float4*>(wr + x) = make_float4(0.f, 0.f, 0.f, 0.f);
+    } else {
+      for (int x = 0; x < td; ++x) wr[x] = 0.0f;
+    }
+    const int lo = max(s, 0), hi = min(s + W, ext);  // tile cells [lo, hi)
+    for (int x = lo; x < hi; ++x) wr[x] = es_eval(off - (float)(x - s), inv_hw, beta);
+    if (lo < hi) hit = (2u << ((hi - 1) / edge)) - (1u << (lo / edge));
+    return hit;
+  }
+  // cell x of the tile is footprint cell (x - st) mod nf, st in (-nf, nf)
+  for (int x = 0; x < td; ++x) {
+    int j = x - st;
+    if (j < 0) j += nf;
+    if (j >= nf) j -= nf;
+    float w = 0.0f;
+    if (x < ext && j < W) {
+      w = es_eval(off - (float)j, inv_hw, beta);
+      hit |= 1u << (x / edge);
+    }
+    wr[x] = w;
+  }
+  return hit;
+}
+
 // Record of an entry for the matrix-core kernel at sp: touched-block mask,
 // G strengths and, per axis, the weight of every tile cell along it (zero
 // outside the footprint and past the grid end).  A lane's weights for a
@@ -406,22 +448,12 @@ __device__ __forceinline__ void write_mfma_entry(float* sp, const BlkPos& r, con
 #pragma unroll
   for (int g = 0; g < G; ++g)
     v[g] = g < Gv ? a.in[(int64_t)(t0 + g) * a.in_stride + r.p] : make_float2(0.f, 0.f);
-  uint32_t hit[3] = {0u, 0u, 0u};
+  uint32_t hit[3];
   float* wr = sp + kBlkHead + 2 * G;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    // cell x of the tile is footprint cell (x - st) mod nf, st in (-nf, nf)
-    for (int x = 0; x < td[d]; ++x) {
-      int j = x - r.st[d];
-      if (j < 0) j += nf[d];
-      if (j >= nf[d]) j -= nf[d];
-      float w = 0.0f;
-      if (x < ext[d] && j < W) {
-        w = es_eval(r.off[d] - (float)j, inv_hw, beta);
-        hit[d] |= 1u << (x / kBlkE);
-      }
-      wr[x] = w;
-    }
+    hit[d] = axis_weights<(kBlkHead + 2 * G) % 4 == 0>(wr, r.st[d], r.off[d], nf[d], td[d],
+                                                       ext[d], W, kBlkE, inv_hw, beta);
     wr += td[d];
   }
   uint32_t mask = 0u;
@@ -784,22 +816,12 @@ __device__ __forceinline__ void write_mfmaz_entry(float* sp, const BlkPos& r,
 #pragma unroll
   for (int g = 0; g < GT; ++g)
     v[g] = g < Gv ? a.in[(int64_t)(t0 + g) * a.in_stride + r.p] : make_float2(0.f, 0.f);
-  uint32_t hit[3] = {0u, 0u, 0u};
+  uint32_t hit[3];
   float* wr = sp + kBlkHead + 2 * GT;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    const int edge = d == 2 ? ZB : kBlkE;
-    for (int x = 0; x < td[d]; ++x) {
-      int j = x - r.st[d];
-      if (j < 0) j += nf[d];
-      if (j >= nf[d]) j -= nf[d];
-      float w = 0.0f;
-      if (x < ext[d] && j < W) {
-        w = es_eval(r.off[d] - (float)j, inv_hw, beta);
-        hit[d] |= 1u << (x / edge);
-      }
-      wr[x] = w;
-    }
+    hit[d] = axis_weights<false>(wr, r.st[d], r.off[d], nf[d], td[d], ext[d], W,
+                                 d == 2 ? ZB : kBlkE, inv_hw, beta);
     wr += td[d];
   }
   uint32_t mask = 0u;
