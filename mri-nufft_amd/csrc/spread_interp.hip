@@ -323,11 +323,17 @@ struct BlkPos {
   int p;
 };
 
+template <int D = 3>
 __device__ __forceinline__ void load_blk_pos(const SpreadArgs& a, int p, const int* org,
                                              BlkPos& r) {
   r.p = p;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
+    r.off[d] = 0.0f;
+    r.st[d] = 0;
+  }
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
     r.off[d] = a.g.off[d][p];
     r.st[d] = a.g.cell[d][p] - org[d];
   }
@@ -438,7 +444,7 @@ __host__ __device__ constexpr int mfma_record_floats(int G, int tiles_sum) {
   return (kBlkHead + 2 * G + tiles_sum + 3) & ~3;
 }
 
-template <int G>
+template <int D, int G>
 __device__ __forceinline__ void write_mfma_entry(float* sp, const BlkPos& r, const SpreadArgs& a,
                                                  int t0, int Gv, const int* td, const int* ext,
                                                  const int* nf, const int* nb, float inv_hw,
@@ -448,12 +454,14 @@ __device__ __forceinline__ void write_mfma_entry(float* sp, const BlkPos& r, con
 #pragma unroll
   for (int g = 0; g < G; ++g)
     v[g] = g < Gv ? a.in[(int64_t)(t0 + g) * a.in_stride + r.p] : make_float2(0.f, 0.f);
-  uint32_t hit[3];
+  uint32_t hit[3] = {1u, 1u, 1u};
   float* wr = sp + kBlkHead + 2 * G;
 #pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    hit[d] = axis_weights<(kBlkHead + 2 * G) % 4 == 0>(wr, r.st[d], r.off[d], nf[d], td[d],
-                                                       ext[d], W, kBlkE, inv_hw, beta);
+  for (int d = 0; d < D; ++d) {
+    // blocks of 4 cells along each axis, 16 along the last in 2D
+    hit[d] = axis_weights<(kBlkHead + 2 * G) % 4 == 0>(
+        wr, r.st[d], r.off[d], nf[d], td[d], ext[d], W,
+        D == 2 && d == 1 ? 4 * kBlkE : kBlkE, inv_hw, beta);
     wr += td[d];
   }
   uint32_t mask = 0u;
@@ -628,12 +636,15 @@ __global__ __launch_bounds__(kThreads) void k_spread_blocks(SpreadArgs a) {
 // both GH: deterministic, equal to the other kernels to rounding.
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-template <int GH, int NWAVE>
+template <int D, int GH, int NWAVE>
 __global__ __launch_bounds__(64 * NWAVE, 4 / GH) void k_spread_mfma(SpreadArgs a) {
   constexpr int G = 8 * GH;              // transforms per workgroup
   constexpr int BPW = 16 / NWAVE;        // blocks per wave (16 blocks at most)
+  // blocks: 4 x 4 x 4 cells in 3D, 4 x 16 in 2D (the last axis holds the
+  // 16 cells of an MFMA row set either way)
+  constexpr int BL = D == 3 ? kBlkE : 4 * kBlkE;  // block edge along the last axis
   extern __shared__ float smem[];
-  const int PW = mfma_record_floats(G, a.g.tile[0] + a.g.tile[1] + a.g.tile[2]);
+  const int PW = mfma_record_floats(G, a.g.tile[0] + a.g.tile[1] + (D == 3 ? a.g.tile[2] : 0));
   const int item = a.item0 + blockIdx.x;
   const int key = a.it_tile[item];
   const int set = key / a.g.nbins;
@@ -644,15 +655,16 @@ __global__ __launch_bounds__(64 * NWAVE, 4 / GH) void k_spread_mfma(SpreadArgs a
   const int Gv = min(G, a.ts - t0);
   const float inv_hw = 2.0f / a.g.w, beta = a.g.beta;
 
-  int tc[3], org[3], ext[3], td[3], nf[3], nb[3];
-  tile_coords<3>(tile, a.g.nt, tc);
+  int tc[3] = {0, 0, 0}, org[3] = {0, 0, 0}, ext[3] = {1, 1, 1}, td[3] = {1, 1, 1},
+      nf[3] = {1, 1, 1}, nb[3] = {1, 1, 1};
+  tile_coords<D>(tile, a.g.nt, tc);
 #pragma unroll
-  for (int d = 0; d < 3; ++d) {
+  for (int d = 0; d < D; ++d) {
     td[d] = a.g.tile[d];
     nf[d] = a.g.nf[d];
     org[d] = tc[d] * td[d];
     ext[d] = min(td[d], nf[d] - org[d]);
-    nb[d] = td[d] / kBlkE;
+    nb[d] = td[d] / (d == D - 1 ? BL : kBlkE);
   }
   const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x / 64);
   const int lane = (int)threadIdx.x & 63;
@@ -669,7 +681,7 @@ __global__ __launch_bounds__(64 * NWAVE, 4 / GH) void k_spread_mfma(SpreadArgs a
     bidx[bi][0] = b / (nb[2] * nb[1]);
     bool live = b < nblk;
 #pragma unroll
-    for (int d = 0; d < 3; ++d) live = live && bidx[bi][d] * kBlkE < ext[d];
+    for (int d = 0; d < D; ++d) live = live && bidx[bi][d] * (d == D - 1 ? BL : kBlkE) < ext[d];
     bbit[bi] = live ? 1u << b : 0u;
   }
 
@@ -696,16 +708,21 @@ __global__ __launch_bounds__(64 * NWAVE, 4 / GH) void k_spread_mfma(SpreadArgs a
 #pragma unroll
   for (int bi = 0; bi < BPW; ++bi) {
     ow4[bi] = kBlkHead + 2 * G + kBlkE * bidx[bi][0];
-    owy[bi] = kBlkHead + 2 * G + td[0] + kBlkE * bidx[bi][1] + cy;
-    owz[bi] = kBlkHead + 2 * G + td[0] + td[1] + kBlkE * bidx[bi][2] + cz;
+    if constexpr (D == 3) {
+      owy[bi] = kBlkHead + 2 * G + td[0] + kBlkE * bidx[bi][1] + cy;
+      owz[bi] = kBlkHead + 2 * G + td[0] + td[1] + kBlkE * bidx[bi][2] + cz;
+    } else {  // the row's cell along the last axis: 16 bidx + 4 cy + cz
+      owy[bi] = kBlkHead + 2 * G + td[0] + BL * bidx[bi][1] + 4 * cy + cz;
+      owz[bi] = 0;
+    }
   }
   const int ov = kBlkHead + (lane & 15);
   for (int eb = e0; eb < e1; eb += CHUNK) {
     const int n = min(CHUNK, e1 - eb);
     if (q < n) {
       BlkPos r;
-      load_blk_pos(a, a.list_pts[eb + q], org, r);
-      write_mfma_entry<G>(smem + q * PW, r, a, t0, Gv, td, ext, nf, nb, inv_hw, beta);
+      load_blk_pos<D>(a, a.list_pts[eb + q], org, r);
+      write_mfma_entry<D, G>(smem + q * PW, r, a, t0, Gv, td, ext, nf, nb, inv_hw, beta);
     }
     __syncthreads();
     int ngroups[BPW];
@@ -739,7 +756,7 @@ __global__ __launch_bounds__(64 * NWAVE, 4 / GH) void k_spread_mfma(SpreadArgs a
         const int on = lst[4 * gq + 4 + kq];  // <= CHUNK + 3: inside the list
         const float* sp = smem + o;
         const float4 w4 = *reinterpret_cast<const float4*>(sp + ow4[bi]);
-        const float wyz = sp[owy[bi]] * sp[owz[bi]];
+        const float wyz = D == 3 ? sp[owy[bi]] * sp[owz[bi]] : sp[owy[bi]];
         float av[kBlkE];
         av[0] = w4.x * wyz;
         av[1] = w4.y * wyz;
@@ -784,18 +801,31 @@ __global__ __launch_bounds__(64 * NWAVE, 4 / GH) void k_spread_mfma(SpreadArgs a
         }
         const int g = 8 * hh + ((lane & 15) >> 1);
         if (g >= Gv) continue;
-        const int c0 = bidx[bi][0] * kBlkE + sx, c1 = bidx[bi][1] * kBlkE + ry;
+        const int c0 = bidx[bi][0] * kBlkE + sx;
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-          const int c2 = bidx[bi][2] * kBlkE + (odd ? 2 : 0) + h;
-          if (c0 >= ext[0] || c1 >= ext[1] || c2 >= ext[2]) continue;
-          if (slot >= 0) {
-            const int r = (c0 * td[1] + c1) * td[2] + c2;
-            a.part[((int64_t)slot * a.ntrans + tb + t0 + g) * TC + r] = out[h];
-          } else {
-            const int64_t gi =
-                ((int64_t)(org[0] + c0) * nf[1] + org[1] + c1) * nf[2] + org[2] + c2;
-            a.grid[(int64_t)(tb + t0 + g) * a.grid_stride + gi] = out[h];
+          if constexpr (D == 3) {
+            const int c1 = bidx[bi][1] * kBlkE + ry;
+            const int c2 = bidx[bi][2] * kBlkE + (odd ? 2 : 0) + h;
+            if (c0 >= ext[0] || c1 >= ext[1] || c2 >= ext[2]) continue;
+            if (slot >= 0) {
+              const int r = (c0 * td[1] + c1) * td[2] + c2;
+              a.part[((int64_t)slot * a.ntrans + tb + t0 + g) * TC + r] = out[h];
+            } else {
+              const int64_t gi =
+                  ((int64_t)(org[0] + c0) * nf[1] + org[1] + c1) * nf[2] + org[2] + c2;
+              a.grid[(int64_t)(tb + t0 + g) * a.grid_stride + gi] = out[h];
+            }
+          } else {  // row cell 4 ry + r of the block's 16 along the last axis
+            const int c1 = bidx[bi][1] * BL + 4 * ry + (odd ? 2 : 0) + h;
+            if (c0 >= ext[0] || c1 >= ext[1]) continue;
+            if (slot >= 0) {
+              const int r = c0 * td[1] + c1;
+              a.part[((int64_t)slot * a.ntrans + tb + t0 + g) * TC + r] = out[h];
+            } else {
+              const int64_t gi = (int64_t)(org[0] + c0) * nf[1] + org[1] + c1;
+              a.grid[(int64_t)(tb + t0 + g) * a.grid_stride + gi] = out[h];
+            }
           }
         }
       }
@@ -2116,11 +2146,38 @@ int launch_spread_sorted(const hipnufft_plan_s* p, const float2* in, SetRange r,
     }
     blocks_ok = blocks_ok && nblk % 4 == 0 && nblk <= 4 * kBlkPerWave;
   }
+  // 2D matrix kernel: blocks of 4 x 16 cells, at most 16 of them per tile
+  const bool mat2d_ok = p->ndim == 2 && p->tile[1] % kBlkE == 0 && p->tile[2] % (4 * kBlkE) == 0 &&
+                        (p->tile[1] / kBlkE) * (p->tile[2] / (4 * kBlkE)) <= 16;
   int method = p->spread_method;
   if (method == HIPNUFFT_SPREAD_AUTO)
     method = (cols_ok && p->spread_group >= 16) ? HIPNUFFT_SPREAD_COLUMNS
              : blocks_ok                        ? HIPNUFFT_SPREAD_MATRIX
+             : (mat2d_ok && r.ts >= 8)          ? HIPNUFFT_SPREAD_MATRIX
                                                 : HIPNUFFT_SPREAD_CELLS;
+  if (p->ndim == 2 && method == HIPNUFFT_SPREAD_MATRIX) {
+    if (!mat2d_ok)
+      return set_error(HIPNUFFT_ERR_ARG,
+                       "2D matrix spreading needs tiles of at most 16 blocks of 4 x 16 cells");
+    // 8 transforms per workgroup, 4 waves of 4 blocks
+    a.G = 8;
+    const size_t chunk = (size_t)kBlkChunk;
+    const size_t lds = sizeof(float) * (chunk + 1) * mfma_record_floats(8, p->tile[1] + p->tile[2]) +
+                       sizeof(int16_t) * 16 * (chunk + 4);
+    if (nitems > 0) {
+      const dim3 grd((unsigned)nitems, (unsigned)((r.ts + 7) / 8));
+      StageScope timer(p, kStageSpread);
+      if ((rc = allow_lds(k_spread_mfma<2, 1, 4>, lds))) return rc;
+      hipLaunchKernelGGL((k_spread_mfma<2, 1, 4>), grd, dim3(256), lds, p->stream, a);
+      HN_LAUNCH_CHECK();
+    }
+    if (ndense > 0) {
+      const dim3 rg((unsigned)ndense, (unsigned)r.ts);
+      hipLaunchKernelGGL(k_spread_reduce<2>, rg, dim3(kThreads), 0, p->stream, a);
+      HN_LAUNCH_CHECK();
+    }
+    return 0;
+  }
   if (method == HIPNUFFT_SPREAD_COLUMNS && !cols_ok)
     return set_error(HIPNUFFT_ERR_ARG,
                      "column spreading needs 3D 8 x 8 x 16 tiles and >= 8 transforms per set");
@@ -2130,7 +2187,7 @@ int launch_spread_sorted(const hipnufft_plan_s* p, const float2* in, SetRange r,
                      "at most 4 along an axis");
   // launches of at most 4 transforms: the matrix kernel with (z cell,
   // transform, re / im) columns, on blocks of 4 x 4 x (8 / GT) cells
-  int GT = r.ts == 1 ? 1 : r.ts == 2 ? 2 : r.ts <= 4 ? 4 : 0;
+  int GT = p->ndim != 3 ? 0 : r.ts == 1 ? 1 : r.ts == 2 ? 2 : r.ts <= 4 ? 4 : 0;
   if (GT) {
     const int zb = 8 / GT;
     const int nblk = (p->tile[0] / kBlkE) * (p->tile[1] / kBlkE) * (p->tile[2] / zb);
@@ -2170,11 +2227,11 @@ int launch_spread_sorted(const hipnufft_plan_s* p, const float2* in, SetRange r,
       const dim3 grd((unsigned)nitems, (unsigned)((r.ts + 8 * GH - 1) / (8 * GH)));
       StageScope timer(p, kStageSpread);
       if (GH == 2) {
-        if ((rc = allow_lds(k_spread_mfma<2, 8>, lds))) return rc;
-        hipLaunchKernelGGL((k_spread_mfma<2, 8>), grd, dim3(2 * kThreads), lds, p->stream, a);
+        if ((rc = allow_lds(k_spread_mfma<3, 2, 8>, lds))) return rc;
+        hipLaunchKernelGGL((k_spread_mfma<3, 2, 8>), grd, dim3(2 * kThreads), lds, p->stream, a);
       } else {
-        if ((rc = allow_lds(k_spread_mfma<1, kSpreadWaves>, lds))) return rc;
-        hipLaunchKernelGGL((k_spread_mfma<1, kSpreadWaves>), grd, dim3(64 * kSpreadWaves), lds,
+        if ((rc = allow_lds(k_spread_mfma<3, 1, kSpreadWaves>, lds))) return rc;
+        hipLaunchKernelGGL((k_spread_mfma<3, 1, kSpreadWaves>), grd, dim3(64 * kSpreadWaves), lds,
                            p->stream, a);
       }
       HN_LAUNCH_CHECK();

@@ -194,6 +194,39 @@ def test_spread_kernels_bit_identical(T):
         assert rel_l2(outs["blocks"][t].cpu().numpy(), exp) < 1e-5
 
 
+@pytest.mark.parametrize("T", [8, 16, 19, 33])
+@pytest.mark.parametrize("tile,split", [(None, 0), (None, 60), ((16, 16), 0), ((8, 64), 90)],
+                         ids=["32x32", "32x32-split", "16x16", "8x64-split"])
+def test_spread_matrix_2d(tile, split, T):
+    """2D matrix-core spreading (blocks of 4 x 16 cells, 8 transforms per
+    workgroup) against the cell-owner kernel (<= 1e-6: same list entries and
+    weights, FP32 sums in another order) and the FP64 oracle (<= 1e-5),
+    with dense tiles split into partial slots, edge tiles past the grid end
+    (33 x 50 image) and a last group of fewer than 8 transforms."""
+    from hipnufft import _capi
+
+    shape = (33, 50)
+    traj = np.concatenate([rand_traj(4000, 2, 61), rand_traj(3000, 2, 62) * 0.05])
+    om = torch.from_numpy((traj * np.float32(2 * np.pi)).astype(np.float32)).to(DEV)
+    y = torch.from_numpy(rand_cplx((T, traj.shape[0]), seed=63)).to(DEV)
+    kw = dict(n_trans=T, spread_max=split)
+    if tile is not None:
+        kw["tile"] = tile
+    outs = {}
+    for m in ("cells", "matrix"):
+        p = _capi.RawPlan(shape, spread_method=m, **kw)
+        p.setpts(om)
+        x = torch.empty((T, *shape), dtype=torch.complex64, device=DEV)
+        p.execute_adjoint(y, x)
+        outs[m] = x.cpu().numpy()
+        p.close()
+    assert rel_l2(outs["matrix"], outs["cells"]) < 1e-6
+    ref = nufft.Plan(om.cpu().numpy(), shape, eps=1e-6)
+    for t in (0, T - 1):
+        exp = ref.execute_adjoint(y[t].cpu().numpy().astype(np.complex128))
+        assert rel_l2(outs["matrix"][t], exp) < 1e-5
+
+
 INTERP_KERNELS = [
     ("stream", {}),                                   # every transform of an item per workgroup
     ("stream-small-items", {"max_subproblem": 40}),   # lp = 4 lanes per point, many items
