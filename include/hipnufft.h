@@ -60,8 +60,8 @@ typedef struct hipnufft_opts {
   int deterministic;    /* reserved (spreading is always fixed-order)        */
   int spread_max;       /* max list entries per spread work item (dense
                            tiles are split and their partials summed;
-                           0: 1024 in 3D; in 2D 256 up to 32 transforms
-                           per launch, else 512; 4096 in 1D)                 */
+                           0: 1024 in 3D; in 2D 256 up to 8 transforms
+                           per plan, 1024 up to 64, else 4096; 4096 in 1D) */
   int spread_method;    /* HIPNUFFT_SPREAD_*: which spreading kernel         */
   int interp_span;      /* bins along the last axis staged together by one
                            interpolation workgroup (0: 2 in 3D, else 1)      */

@@ -574,12 +574,13 @@ int hipnufft_plan_create(int type, int ndim, const int64_t* nmodes, int iflag, i
   p->interp_group = o.interp_group > 0 ? o.interp_group : (ndim == 3 ? 0 : 4);
   p->max_sub = o.max_subproblem > 0 ? o.max_subproblem : (ndim == 3 ? 128 : 1024);
   // dense tiles (a radial k-space centre holds ~80 K list entries per tile)
-  // split finely, so that their work items do not form a tail: 1024 entries
-  // in 3D; in 2D 256 for up to 32 transforms per launch (radial 256^2,
-  // 128 x 512 samples, 8 coils: op+adj 0.60 ms at 1024, 0.38 at 512, 0.26 at
-  // 256) and 512 above, where the partial tiles' reduction grows with the
-  // transforms (1024 stacked planes: 9.26 ms at 256, 8.67 at 512)
-  const int split2d = ntrans <= 32 ? 256 : 512;
+  // split, so that their work items do not form a tail: 1024 entries in 3D.
+  // In 2D (matrix-core spreading from 8 transforms per launch) the best
+  // split grows with the transforms, whose partial tiles the reduction sums:
+  // op+adj of radial 256^2, 128 x 512 samples, 8 coils 0.18 ms at 256, 0.23
+  // at 1024; 256 x 512 at 32 coils 0.69 ms at 256, 0.58 at 1024; 1024
+  // stacked planes 4.23 ms at 512, 3.76 at 4096, 6.25 at 8192
+  const int split2d = ntrans <= 8 ? 256 : ntrans <= 64 ? 1024 : 4096;
   p->spread_max = o.spread_max > 0 ? o.spread_max : (ndim == 3 ? 1024 : ndim == 2 ? split2d : 4096);
   if (o.spread_method < HIPNUFFT_SPREAD_AUTO || o.spread_method > HIPNUFFT_SPREAD_MATRIX) {
     free_plan(p);
