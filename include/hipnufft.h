@@ -52,11 +52,13 @@ typedef struct hipnufft_opts {
   int tile[3];          /* bin / tile edge per axis (powers of two)          */
   int spread_group;     /* transforms accumulated together in one spread WG */
   int interp_group;     /* transforms gathered together in one interp WG
-                           (0 in 3D: all of a work item's transforms, the
-                           region streamed through registers; >= 1: that
-                           many regions staged side by side)                */
+                           (<= 0 in 2D / 3D: all of a work item's
+                           transforms in one WG, the region streamed
+                           through LDS; >= 1: that many regions staged side
+                           by side; 0 in 1D: 4)                             */
   int max_subproblem;   /* max points per interp work item (dense bins split;
-                           0: 128 in 3D, else 1024)                         */
+                           0: 128 in 3D, 256 in 2D when streaming, else
+                           1024)                                            */
   int deterministic;    /* reserved (spreading is always fixed-order)        */
   int spread_max;       /* max list entries per spread work item (dense
                            tiles are split and their partials summed;

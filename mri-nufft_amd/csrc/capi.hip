@@ -568,11 +568,15 @@ int hipnufft_plan_create(int type, int ndim, const int64_t* nmodes, int iflag, i
     return set_error(HIPNUFFT_ERR_ARG, "too many tiles");
   }
   p->spread_group = o.spread_group > 0 ? o.spread_group : 0;  // 0: chosen per launch
-  // 3D: 0 = every transform of a work item in one workgroup (items of up
-  // to 128 points: more, smaller items measured faster at 16 coils than 256
-  // or 512); an explicit group picks the per-transform kernel
-  p->interp_group = o.interp_group > 0 ? o.interp_group : (ndim == 3 ? 0 : 4);
-  p->max_sub = o.max_subproblem > 0 ? o.max_subproblem : (ndim == 3 ? 128 : 1024);
+  // 2D / 3D: 0 = every transform of a work item in one workgroup (items of
+  // up to 128 points in 3D: more, smaller items measured faster at 16 coils
+  // than 256 or 512; 256 in 2D, faster than 128 for radial 256^2 at 8 and
+  // 64 coils and for the stacked 128^3 slices); an explicit group picks the
+  // kernel that stages that many transforms side by side (1024 points)
+  p->interp_group = o.interp_group > 0 ? o.interp_group : ndim == 1 ? 4 : 0;
+  p->max_sub = o.max_subproblem > 0 ? o.max_subproblem
+               : ndim == 3              ? 128
+               : ndim == 2 && p->interp_group == 0 ? 256 : 1024;
   // dense tiles (a radial k-space centre holds ~80 K list entries per tile)
   // split, so that their work items do not form a tail: 1024 entries in 3D.
   // In 2D (matrix-core spreading from 8 transforms per launch) the best

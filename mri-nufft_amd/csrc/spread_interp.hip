@@ -1635,8 +1635,15 @@ constexpr int kStreamCells = 9728;
 constexpr int kStreamRows = 256;
 constexpr int kStreamMaxW = 8;
 constexpr int kStreamPad = 0;
+// 2D regions (32 x 32 tiles plus the halo: 1521 cells) are small, so 2D
+// workgroups of 256 threads hold 2048 cells and several share a CU
+constexpr int kStream2NT = 256;
+constexpr int kStream2Cells = 2048;
+constexpr int kStream2Rows = 128;
 
-// 3D interpolation of many transforms of a work item in one workgroup.
+// 2D / 3D interpolation of many transforms of a work item in one workgroup
+// (in 2D a region row is one line along the contiguous axis, in 3D one
+// (x, y) pair's line).
 // What does not depend on the transform is done once per item: the grid
 // byte offset of every staged 32-bit word of the region (each thread owns
 // the words tid + u * NT, u < K; a word of a chunk no point reads gets an
@@ -1652,13 +1659,13 @@ constexpr int kStreamPad = 0;
 // the kernel rows j0 = s, s + lp, ...; the partial sums meet by lane
 // exchanges in a fixed order, so the result does not depend on the run.
 // The footprint gathers bound the kernel: random 16-byte LDS reads.
-template <int W, int NT, int NB>
+template <int D, int W, int NT, int NB, int CELLS, int ROWS>
 __global__ __launch_bounds__(NT, 4) void k_interp_stream(InterpArgs a) {
-  // static LDS (NB buffers and the row tables): the compiler then knows how
-  // many workgroups share a CU
-  __shared__ float2 lds[NB * kStreamCells + kStreamRows];
-  constexpr int K = 2 * kStreamCells / NT;  // staged words per thread
-  constexpr int D = 3;
+  // static LDS (NB buffers of CELLS cells and the row tables): the compiler
+  // then knows how many workgroups share a CU
+  __shared__ float2 lds[NB * CELLS + ROWS];
+  constexpr int K = 2 * CELLS / NT;  // staged words per thread
+  constexpr int DL = D - 1;                 // the contiguous axis
   constexpr uint32_t kSkip = 0x80000000u;
   constexpr int LGMIN = NT >= 1024 ? 1 : 0;  // items hold at most NT >> LGMIN points
   constexpr int KJ = (W + (1 << LGMIN) - 1) >> LGMIN;  // rows per lane at the fewest lanes
@@ -1673,8 +1680,8 @@ __global__ __launch_bounds__(NT, 4) void k_interp_stream(InterpArgs a) {
   const int tb = (set - a.k0) * a.ts + t0;  // chunk transform of the first handled here
   int rorg[3], R[3];
   region_of<D, W>(a, tile, rorg, R);
-  const int NR = R[0] * R[1];
-  const int RL = R[2] + a.lds_pad;  // LDS row length: the padding moves rows across banks
+  const int NR = D == 3 ? R[0] * R[1] : R[0];  // region rows along the contiguous axis
+  const int RL = R[DL] + a.lds_pad;  // LDS row length: the padding moves rows across banks
   const int RC = NR * RL;
   const int RCP = (RC + 31) & ~31;  // a buffer: whole 64-word wave segments
   const int tid = (int)threadIdx.x;
@@ -1684,14 +1691,18 @@ __global__ __launch_bounds__(NT, 4) void k_interp_stream(InterpArgs a) {
   int* rowoff = reinterpret_cast<int*>(lds + NB * RCP);
   uint32_t* rowmask = reinterpret_cast<uint32_t*>(rowoff + NR);
   for (int r = tid; r < NR; r += NT) {
-    const int l0 = mdiv(r, a.rmag[1]);
-    const int l1 = r - l0 * R[1];
-    rowoff[r] = (wrap_coord(rorg[0] + l0, a.g.nf[0], a.wrap1) * a.g.nf[1] +
-                 wrap_coord(rorg[1] + l1, a.g.nf[1], a.wrap1)) * a.g.nf[2];
+    if constexpr (D == 3) {
+      const int l0 = mdiv(r, a.rmag[1]);
+      const int l1 = r - l0 * R[1];
+      rowoff[r] = (wrap_coord(rorg[0] + l0, a.g.nf[0], a.wrap1) * a.g.nf[1] +
+                   wrap_coord(rorg[1] + l1, a.g.nf[1], a.wrap1)) * a.g.nf[2];
+    } else {
+      rowoff[r] = wrap_coord(rorg[0] + r, a.g.nf[0], a.wrap1) * a.g.nf[1];
+    }
     rowmask[r] = a.sub_mask ? (uint32_t)a.sub_mask[(int64_t)sub * NR + r] : 0xffffffffu;
   }
   __syncthreads();
-  const int phase = ((rorg[2] % 16) + 16) % 16;
+  const int phase = ((rorg[DL] % 16) + 16) % 16;
   uint32_t off[K];
 #pragma unroll
   for (int u = 0; u < K; ++u) {
@@ -1701,8 +1712,8 @@ __global__ __launch_bounds__(NT, 4) void k_interp_stream(InterpArgs a) {
     if (c < RC) {
       const int r = mdiv(c, a.lds_row_mag);
       const int z = c - r * RL;
-      if (z < R[2] && ((rowmask[r] >> ((z + phase) >> 4)) & 1u))
-        off[u] = (uint32_t)(rowoff[r] + wrap_coord(rorg[2] + z, a.g.nf[2], a.wrap1)) * 8u +
+      if (z < R[DL] && ((rowmask[r] >> ((z + phase) >> 4)) & 1u))
+        off[u] = (uint32_t)(rowoff[r] + wrap_coord(rorg[DL] + z, a.g.nf[DL], a.wrap1)) * 8u +
                  4u * (uint32_t)(wd & 1);
     }
   }
@@ -1728,22 +1739,22 @@ __global__ __launch_bounds__(NT, 4) void k_interp_stream(InterpArgs a) {
     float wt[D][W];
     point_setup<D, W>(a, p, rorg, ls, wt);
 #pragma unroll
-    for (int j = 0; j < W; ++j) {
-      wt1[j] = wt[1][j];
-      wt2[j] = wt[2][j];
+    for (int j = 0; j < W; ++j) {  // wt2: the contiguous axis; wt1: the middle one (3D)
+      wt1[j] = wt[D == 3 ? 1 : 0][j];
+      wt2[j] = wt[DL][j];
     }
     const float inv_hw = 2.0f / W;
     const float off0 = a.g.off[0][p];
 #pragma unroll
     for (int k = 0; k < KJ; ++k)  // the weights of the lane's rows s + k lp
       w0[k] = s + k * lp < W ? es_eval(off0 - (float)(s + k * lp), inv_hw, a.g.beta) : 0.0f;
-    lbase = ((ls[0] + s) * R[1] + ls[1]) * RL + ls[2];
+    lbase = D == 3 ? ((ls[0] + s) * R[1] + ls[1]) * RL + ls[2] : (ls[0] + s) * RL + ls[1];
     const int idx = a.g.idx[p];
     dens = (a.mode == 1 && a.density) ? a.density[idx] : 1.0f;
     kbase = ksp_index(a.g, a.ts, set, 0, idx);
     kstep = ksp_index(a.g, a.ts, set, 1, idx) - kbase;
   }
-  const int rstep = lp * R[1] * RL;  // LDS cells between the lane's rows
+  const int rstep = lp * (D == 3 ? R[1] : 1) * RL;  // LDS cells between the lane's rows
 
   // the region of transform t into buffer b: load u of wave w fills words
   // [u NT + 64 w, u NT + 64 w + 64) of the buffer
@@ -1786,18 +1797,27 @@ __global__ __launch_bounds__(NT, 4) void k_interp_stream(InterpArgs a) {
       if (live && s + k * lp < W) {
         const float2* rowk = cur + lbase + k * rstep;
         float px = 0.f, py = 0.f;
+        if constexpr (D == 3) {
 #pragma unroll
-        for (int j1 = 0; j1 < W; ++j1) {
-          const float2* row = rowk + j1 * RL;
-          float rx = 0.f, ry = 0.f;
+          for (int j1 = 0; j1 < W; ++j1) {
+            const float2* row = rowk + j1 * RL;
+            float rx = 0.f, ry = 0.f;
+#pragma unroll
+            for (int j2 = 0; j2 < W; ++j2) {
+              const float2 c = row[j2];
+              rx += wt2[j2] * c.x;
+              ry += wt2[j2] * c.y;
+            }
+            px += wt1[j1] * rx;
+            py += wt1[j1] * ry;
+          }
+        } else {  // 2D: row s + k lp of the footprint is one line along the contiguous axis
 #pragma unroll
           for (int j2 = 0; j2 < W; ++j2) {
-            const float2 c = row[j2];
-            rx += wt2[j2] * c.x;
-            ry += wt2[j2] * c.y;
+            const float2 c = rowk[j2];
+            px += wt2[j2] * c.x;
+            py += wt2[j2] * c.y;
           }
-          px += wt1[j1] * rx;
-          py += wt1[j1] * ry;
         }
         sx += w0[k] * px;
         sy += w0[k] * py;
@@ -2373,38 +2393,58 @@ int launch_interp(const hipnufft_plan_s* p, const float2* grid, int64_t grid_str
   a.mode = mode;
   if (region_lds_bytes(p, 1) > kMaxLdsRegion)
     return set_error(HIPNUFFT_ERR_ARG, "interp tile too large for LDS");
-  if (p->ndim == 3 && p->interp_group <= 0) {
+  if (p->ndim >= 2 && p->interp_group <= 0) {
     // every transform of an item in one workgroup (k_interp_stream) when
     // its region, grid and items fit the kernel
-    constexpr int NT = kStreamNT, NB = kStreamNB;
+    constexpr int NB = kStreamNB;
+    const bool d3 = p->ndim == 3;
+    const int NT = d3 ? kStreamNT : kStream2NT;
     int64_t cells = 1;
-    for (int d = 0; d < 3; ++d) cells *= region_extent(p, d);
+    for (int d = 3 - p->ndim; d < 3; ++d) cells *= region_extent(p, d);
     const int64_t rows = cells / region_extent(p, 2);
     const int64_t padded = (rows * (region_extent(p, 2) + kStreamPad) + 31) / 32 * 32;
-    if (padded <= kStreamCells && rows <= kStreamRows && grid_size(p) < ((int64_t)1 << 28) &&
-        p->max_sub <= (NT >= 1024 ? NT / 2 : NT) && p->w <= kStreamMaxW) {
+    if (padded <= (d3 ? kStreamCells : kStream2Cells) && rows <= (d3 ? kStreamRows : kStream2Rows) &&
+        grid_size(p) < ((int64_t)1 << 28) && p->max_sub <= (NT >= 1024 ? NT / 2 : NT) &&
+        p->w <= kStreamMaxW) {
       set_region_magic(p, &a, NT);
       a.lds_pad = kStreamPad;
       a.lds_row_mag = magic_of(region_extent(p, 2) + kStreamPad);
       a.grid_bytes = (int)(grid_size(p) * (int64_t)sizeof(float2));
       a.sub_mask = p->mask_rows > 0 ? p->sub_mask : nullptr;
       // transforms per workgroup: all of the set's, fewer when the items
-      // alone would not fill the GPU several times over
+      // alone would not fill the GPU several times over.  3D: at least 1024
+      // workgroups (two per CU).  2D (eight per CU): at least 16384, dense
+      // items then leave no tail (stacked 128^3 slices, 2048 transforms:
+      // 0.37 ms less per type 2 than with 4096), but at least 16 transforms
+      // each, over which the point set-up is shared (radial 256^2, 64 coils:
+      // 0.228 ms at 4 transforms, 0.195 at 16)
       int Gs = r.ts;
-      while (Gs > 1 && nsub * (int64_t)((r.ts + Gs - 1) / Gs) < 1024) Gs = (Gs + 1) / 2;
+      if (d3) {
+        while (Gs > 1 && nsub * (int64_t)((r.ts + Gs - 1) / Gs) < 1024) Gs = (Gs + 1) / 2;
+      } else {
+        while (Gs > 16 && nsub * (int64_t)((r.ts + Gs - 1) / Gs) < 16384) Gs = (Gs + 1) / 2;
+      }
       a.G = Gs;
       const dim3 grd((unsigned)nsub, (unsigned)((r.ts + Gs - 1) / Gs));
       StageScope timer(p, kStageInterp);
       HN_WIDTH_SWITCH(p->w, {
         if constexpr (WW <= kStreamMaxW) {
-          hipLaunchKernelGGL((k_interp_stream<WW, NT, NB>), grd, dim3(NT), 0, p->stream, a);
+          if (d3)
+            hipLaunchKernelGGL((k_interp_stream<3, WW, kStreamNT, NB, kStreamCells, kStreamRows>),
+                               grd, dim3(NT), 0, p->stream, a);
+          else
+            hipLaunchKernelGGL((k_interp_stream<2, WW, kStream2NT, NB, kStream2Cells, kStream2Rows>),
+                               grd, dim3(NT), 0, p->stream, a);
         }
       });
       HN_LAUNCH_CHECK();
       return 0;
     }
   }
-  int G = std::max(1, std::min(p->interp_group, r.ts));
+  // regions that do not fit the streaming kernel: one transform per
+  // workgroup in 3D, four in 2D
+  const int g0 = p->interp_group > 0 ? p->interp_group : p->ndim == 2 ? 4 : 1;
+  int G = std::max(1, std::min(g0, r.ts));
   while (G > 1 && region_lds_bytes(p, G) > kMaxLds) --G;
   a.G = G;
   const size_t lds = region_lds_bytes(p, G);

@@ -148,10 +148,11 @@ def autotune(coords, shape, n_trans=1, eps=1e-6, upsampfac=2.0, reps=3, budget_s
         ("tile", tile_candidates(ndim, w)),
         ("spread_method", [1, 2, 4] if ndim == 3 else []),
         ("spread_group", [4, 8, 16]),
-        # 3D: 0 interpolates every transform of a work item in one workgroup
-        # (items of at most 512 points), 1 one transform per workgroup
-        ("interp_group", [1, 2, 4] if ndim < 3 else [1]),
-        ("max_subproblem", [512, 1024, 2048] if ndim < 3 else [64, 256, 1024]),
+        # 2D / 3D: 0 interpolates every transform of a work item in one
+        # workgroup (items of at most 256 points in 2D, 512 in 3D), g >= 1
+        # stages g transforms side by side; 1D: 4 by default
+        ("interp_group", {1: [1, 2], 2: [1, 4], 3: [1]}[ndim]),
+        ("max_subproblem", {1: [512, 1024, 2048], 2: [128, 512, 1024], 3: [64, 256, 1024]}[ndim]),
         ("spread_max", [256, 512, 1024, 2048, 4096]),
     ]
     for name, values in stages:

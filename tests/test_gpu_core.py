@@ -268,6 +268,46 @@ def test_interp_kernels_3d(shape, T):
         assert rel_l2(outs[name], outs["per-transform"]) < 1e-6, name
 
 
+INTERP_KERNELS_2D = [
+    ("stream", {}),                                   # every transform of an item per workgroup
+    ("stream-small-items", {"interp_group": -1, "max_subproblem": 40}),
+    ("stream-span2", {"max_subproblem": 200, "tile": (16, 16), "interp_span": 2}),
+    ("grouped", {"interp_group": 4}),                 # 4 transforms per workgroup
+    ("fallback-big-items", {"max_subproblem": 700}),  # items too large for the stream kernel
+    ("per-transform", {"interp_group": 1}),
+]
+
+
+@pytest.mark.parametrize("T", [1, 3, 16, 19])
+@pytest.mark.parametrize("shape", [(72, 40), (6, 5)], ids=["grid", "tiny-wrapping"])
+def test_interp_kernels_2d(shape, T):
+    """2D type 2 through the streaming kernel (all transforms of a work item
+    in one workgroup, the default) and the kernel that stages 1 or 4
+    transforms side by side: each transform against the
+    FP64 oracle NUFFT (<= 1e-5), and the kernels against each other
+    (<= 1e-6).  The tiny grid is smaller than the kernel footprint."""
+    from hipnufft import _capi
+
+    traj = np.concatenate([rand_traj(4000, 2, 61), rand_traj(3000, 2, 62) * 0.05])
+    om = torch.from_numpy((traj * np.float32(2 * np.pi)).astype(np.float32)).to(DEV)
+    x = torch.from_numpy(rand_cplx((T, *shape), seed=63)).to(DEV)
+    outs = {}
+    for name, kw in INTERP_KERNELS_2D:
+        p = _capi.RawPlan(shape, n_trans=T, **kw)
+        p.setpts(om)
+        y = torch.empty((T, traj.shape[0]), dtype=torch.complex64, device=DEV)
+        p.execute(x, y)
+        outs[name] = y.cpu().numpy()
+        p.close()
+    ref = nufft.Plan(om.cpu().numpy(), shape, eps=1e-6)
+    for t in sorted({0, T - 1}):
+        exp = ref.execute(x[t].cpu().numpy().astype(np.complex128))
+        for name in outs:
+            assert rel_l2(outs[name][t], exp) < 1e-5, (name, t)
+    for name in outs:
+        assert rel_l2(outs[name], outs["per-transform"]) < 1e-6, name
+
+
 @pytest.mark.parametrize("T", [5, 12])
 @pytest.mark.parametrize("tile,shape,eps", [
     ((8, 8, 8), (16, 12, 10), 1e-6),     # fine grid 32 x 24 x 20: partial edge tiles
