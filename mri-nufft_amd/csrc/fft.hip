@@ -344,9 +344,14 @@ __global__ __launch_bounds__(kThreads) void k_rows_fwd(RowsArgs a) {
   const int L = 1 << a.lgL, LP = L + 1;
   const float2* twl = stage_twiddles(buf, a.ax, LP);
   const int rows = a.M0 * a.M1;
+  // transforms fastest in the remapped order: the workgroups that read one
+  // block of image rows (all coils of an image) run together on one XCD, so
+  // the image comes from HBM once rather than once per coil (192^3, 16
+  // coils: 0.809 -> 0.788 ms)
   const BlockIdx3 bi = remapped_block();
-  const int r0 = bi.x * L;
-  const int t = bi.y;
+  const int lin = bi.x + (int)gridDim.x * bi.y;
+  const int t = lin % (int)gridDim.y;
+  const int r0 = (lin / (int)gridDim.y) * L;
   const int tt = a.t0 + t;
   const int64_t N = (int64_t)rows * a.M2;
   const int n = a.nf2;
