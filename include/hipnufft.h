@@ -75,8 +75,8 @@ typedef struct hipnufft_opts {
  * result on every run, equal to the others to FP32 rounding).  AUTO:
  * COLUMNS for 3D 8 x 8 x 16 tiles when spread_group >= 16 and a set has
  * >= 8 transforms, else MATRIX for 3D tiles with edges of 4, 8 or 16 cells
- * (4, 8 or 16 blocks of 4 x 4 x 4 cells) and for 2D launches of >= 8
- * transforms on tiles of at most 16 blocks of 4 x 16 cells, else CELLS. */
+ * (4, 8 or 16 blocks of 4 x 4 x 4 cells) and for 2D tiles of at most 16
+ * blocks of 4 x 16 cells, else CELLS. */
 enum {
   HIPNUFFT_SPREAD_AUTO = 0,
   HIPNUFFT_SPREAD_CELLS = 1,   /* thread per (column, 4-cell segment)   */

@@ -579,7 +579,7 @@ int hipnufft_plan_create(int type, int ndim, const int64_t* nmodes, int iflag, i
                : ndim == 2 && p->interp_group == 0 ? 256 : 1024;
   // dense tiles (a radial k-space centre holds ~80 K list entries per tile)
   // split, so that their work items do not form a tail: 1024 entries in 3D.
-  // In 2D (matrix-core spreading from 8 transforms per launch) the best
+  // In 2D (matrix-core spreading) the best
   // split grows with the transforms, whose partial tiles the reduction sums:
   // op+adj of radial 256^2, 128 x 512 samples, 8 coils 0.18 ms at 256, 0.23
   // at 1024; 256 x 512 at 32 coils 0.69 ms at 256, 0.58 at 1024; 1024

@@ -2173,7 +2173,7 @@ int launch_spread_sorted(const hipnufft_plan_s* p, const float2* in, SetRange r,
   if (method == HIPNUFFT_SPREAD_AUTO)
     method = (cols_ok && p->spread_group >= 16) ? HIPNUFFT_SPREAD_COLUMNS
              : blocks_ok                        ? HIPNUFFT_SPREAD_MATRIX
-             : (mat2d_ok && r.ts >= 8)          ? HIPNUFFT_SPREAD_MATRIX
+             : mat2d_ok                         ? HIPNUFFT_SPREAD_MATRIX
                                                 : HIPNUFFT_SPREAD_CELLS;
   if (p->ndim == 2 && method == HIPNUFFT_SPREAD_MATRIX) {
     if (!mat2d_ok)

@@ -194,7 +194,7 @@ def test_spread_kernels_bit_identical(T):
         assert rel_l2(outs["blocks"][t].cpu().numpy(), exp) < 1e-5
 
 
-@pytest.mark.parametrize("T", [8, 16, 19, 33])
+@pytest.mark.parametrize("T", [1, 3, 8, 16, 19, 33])
 @pytest.mark.parametrize("tile,split", [(None, 0), (None, 60), ((16, 16), 0), ((8, 64), 90)],
                          ids=["32x32", "32x32-split", "16x16", "8x64-split"])
 def test_spread_matrix_2d(tile, split, T):
@@ -202,7 +202,8 @@ def test_spread_matrix_2d(tile, split, T):
     workgroup) against the cell-owner kernel (<= 1e-6: same list entries and
     weights, FP32 sums in another order) and the FP64 oracle (<= 1e-5),
     with dense tiles split into partial slots, edge tiles past the grid end
-    (33 x 50 image) and a last group of fewer than 8 transforms."""
+    (33 x 50 image), launches of 1 or 3 transforms and a last group of fewer
+    than 8 transforms."""
     from hipnufft import _capi
 
     shape = (33, 50)
