@@ -896,6 +896,7 @@ int strided(const hipnufft_plan_s* p, int axis, int T, int in_M, int out_M, int 
   HN_STRIDED_FIXED(384, 4, 2 * kThreads, 12)
   HN_STRIDED_FIXED(256, 4, kThreads, 16)
   HN_STRIDED_FIXED(512, 3, kThreads, 16)
+  HN_STRIDED_FIXED(768, 3, 2 * kThreads, 12)
 #undef HN_STRIDED_FIXED
   if (ax.n * ax.L > kMaxElemsFFT) {
     HN_EPT_SWITCH(ept_of(ax, 2 * kThreads), {
@@ -1015,6 +1016,11 @@ int rows_fwd(const hipnufft_plan_s* p, const float2* img, const float2* smaps, i
         hipLaunchKernelGGL((k_rows_fwd<-1, 16, 512, 3>), grd, dim3(kThreads), lds, p->stream, a);
       else
         hipLaunchKernelGGL((k_rows_fwd<1, 16, 512, 3>), grd, dim3(kThreads), lds, p->stream, a);
+    } else if (fixed_plan(ax, 768, 4)) {
+      if (sign < 0)
+        hipLaunchKernelGGL((k_rows_fwd<-1, 12, 768, 2>), grd, dim3(kThreads), lds, p->stream, a);
+      else
+        hipLaunchKernelGGL((k_rows_fwd<1, 12, 768, 2>), grd, dim3(kThreads), lds, p->stream, a);
     } else {
       HN_EPT_SWITCH(ept_of(p->fft[2]), {
         if (sign < 0)
@@ -1063,6 +1069,11 @@ int rows_adj(const hipnufft_plan_s* p, float2* img, const float2* smaps, int C, 
       hipLaunchKernelGGL((k_rows_adj<-1, 16, 512, 3>), grd, dim3(kThreads), lds, p->stream, a);
     else
       hipLaunchKernelGGL((k_rows_adj<1, 16, 512, 3>), grd, dim3(kThreads), lds, p->stream, a);
+  } else if (fixed_plan(ax, 768, 4)) {
+    if (sign < 0)
+      hipLaunchKernelGGL((k_rows_adj<-1, 12, 768, 2>), grd, dim3(kThreads), lds, p->stream, a);
+    else
+      hipLaunchKernelGGL((k_rows_adj<1, 12, 768, 2>), grd, dim3(kThreads), lds, p->stream, a);
   } else {
     HN_EPT_SWITCH(ept_of(p->fft[2]), {
       if (sign < 0)

@@ -1,4 +1,4 @@
-"""Parity at the headline configuration and on 192-point grid axes.
+"""Parity at the headline configuration and on 192- and 384-point image axes.
 
 Headline: 3D golden-angle phyllotaxis radial, 4096 spokes x 512 samples
 (2,097,152 points), image 192^3 (ellipsoid phantom), 16 Gaussian-bump SENSE
@@ -146,10 +146,12 @@ def test_reduced_3d_cg_vs_oracle_cg(density):
     assert err < 1e-4
 
 
-@pytest.mark.parametrize("shape,C", [((192, 192), 2), ((8, 192, 192), 3)])
+@pytest.mark.parametrize("shape,C", [((192, 192), 2), ((8, 192, 192), 3), ((384, 384), 2),
+                                     ((4, 384, 384), 2)])
 def test_192_point_axes_vs_ndft(shape, C):
-    """384-point fine-grid axes (compile-time row and strided FFT instances)
-    at small size: op / adj_op against the exact FP64 NDFT and adjointness."""
+    """384- and 768-point fine-grid axes (compile-time row and strided FFT
+    instances) at small size: op / adj_op against the exact FP64 NDFT and
+    adjointness."""
     from hipnufft import MRIHipNUFFT
 
     d = len(shape)
@@ -157,7 +159,7 @@ def test_192_point_axes_vs_ndft(shape, C):
     smaps = gauss_smaps(C, shape, seed=22)
     op = MRIHipNUFFT(traj, shape, n_coils=C, smaps=smaps)
     nf = tuple(op.raw_op.plan.info()["nf"])[-d:]
-    assert all(n == 384 for n, s in zip(nf, shape) if s == 192), nf
+    assert all(n == 2 * s for n, s in zip(nf, shape) if s in (192, 384)), nf
     x = torch.from_numpy(rand_cplx(shape, 23)).to(DEV)
     y = op.op(x).reshape(C, -1)
     yy = torch.from_numpy(rand_cplx((1, C, len(traj)), 24)).to(DEV)
