@@ -309,6 +309,37 @@ def test_interp_kernels_2d(shape, T):
         assert rel_l2(outs[name], outs["per-transform"]) < 1e-6, name
 
 
+@pytest.mark.parametrize("eps,upsampfac", [(1e-2, 2.0), (1e-4, 2.0), (1e-4, 1.25), (1e-6, 1.5)])
+@pytest.mark.parametrize("shape", [(40, 36), (12, 10, 14)], ids=["2d", "3d"])
+def test_eps_and_upsampfac_vs_oracle(shape, eps, upsampfac):
+    """Kernel widths 3, 5, 7 and 8 (eps 1e-2 .. 1e-6, oversampling 2, 1.25
+    and 1.5) through the default kernels: the plan's fine grid and width
+    equal the oracle plan's of the same eps and oversampling, and type 2 and
+    type 1 of three transforms agree with it to FP32 rounding (<= 1e-5)."""
+    from hipnufft import _capi
+
+    d = len(shape)
+    T = 3
+    traj = np.concatenate([rand_traj(3000, d, 71), rand_traj(1500, d, 72) * 0.1])
+    om = torch.from_numpy((traj * np.float32(2 * np.pi)).astype(np.float32)).to(DEV)
+    ref = nufft.Plan(om.cpu().numpy(), shape, eps=eps, upsampfac=upsampfac)
+    p = _capi.RawPlan(shape, n_trans=T, eps=eps, upsampfac=upsampfac)
+    p.setpts(om)
+    info = p.info()
+    assert tuple(info["nf"])[-d:] == tuple(ref.nf) and info["w"] == ref.w, (info, ref.nf, ref.w)
+    x = rand_cplx((T, *shape), 73)
+    c = rand_cplx((T, traj.shape[0]), 74)
+    y = torch.empty((T, traj.shape[0]), dtype=torch.complex64, device=DEV)
+    z = torch.empty((T, *shape), dtype=torch.complex64, device=DEV)
+    p.execute(torch.from_numpy(x).to(DEV), y)
+    p.execute_adjoint(torch.from_numpy(c).to(DEV), z)
+    p.close()
+    y, z = y.cpu().numpy(), z.cpu().numpy()
+    for t in (0, T - 1):
+        assert rel_l2(y[t], ref.execute(x[t])) < 1e-5, t
+        assert rel_l2(z[t], ref.execute_adjoint(c[t])) < 1e-5, t
+
+
 @pytest.mark.parametrize("T", [5, 12])
 @pytest.mark.parametrize("tile,shape,eps", [
     ((8, 8, 8), (16, 12, 10), 1e-6),     # fine grid 32 x 24 x 20: partial edge tiles
