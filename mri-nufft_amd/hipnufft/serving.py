@@ -129,24 +129,45 @@ def stack_layout(samples, shape):
     None otherwise (and for 2D shapes)."""
     if len(shape) != 3:
         return None
-    s = np.asarray(samples, dtype=np.float64).reshape(-1, 3)
+    # the checks run on the samples' own dtype (exact: equality, differences
+    # of neighbours and the maximum magnitude do not change in float64);
+    # only the tolerance test for planes that are not exact copies, and the
+    # few per-plane values, are computed in float64
+    s = np.asarray(samples).reshape(-1, 3)
     if len(s) == 0:
         return None
-    if np.max(np.abs(s)) - 1e-4 > 0.5:      # radians
-        s = s / (2 * np.pi)
+    if s.dtype not in (np.float32, np.float64):
+        s = s.astype(np.float64)
+    s = np.ascontiguousarray(s)
+    rad = max(float(s.max()), -float(s.min())) - 1e-4 > 0.5  # radians
+
+    def unit(a):  # float64, in [-0.5, 0.5)
+        a = a.astype(np.float64)
+        return a / (2 * np.pi) if rad else a
+
     Z = int(shape[2])
-    NZ = int(np.count_nonzero(np.diff(s[:, 2]))) + 1
+    z = s[:, 2]
+    NZ = int(np.count_nonzero(z[1:] != z[:-1])) + 1
     if len(s) % NZ:
         return None
     planes = s.reshape(NZ, len(s) // NZ, 3)
-    if not (np.all(planes[:, :, 2] == planes[:, :1, 2])
-            and np.allclose(planes[:, :, :2], planes[:1, :, :2], rtol=0.0, atol=1e-7)):
+    if not np.all(planes[:, :, 2] == planes[:, :1, 2]):
         return None
-    zf = planes[:, 0, 2] * Z + Z // 2
+    # (x, y) of every sample read as one complex number (the first two
+    # coordinates of a row are adjacent): one comparison per sample, with
+    # the floats' own equality
+    isz = s.dtype.itemsize
+    xyc = np.ndarray(shape=planes.shape[:2], dtype=np.complex64 if isz == 4 else np.complex128,
+                     buffer=s, strides=(planes.shape[1] * 3 * isz, 3 * isz))
+    if not bool((xyc == xyc[:1]).all()):
+        xy64 = unit(planes[:, :, :2])
+        if not np.allclose(xy64, xy64[:1], rtol=0.0, atol=1e-7):
+            return None
+    zf = unit(planes[:, 0, 2]) * Z + Z // 2
     zi = np.rint(zf)
     if np.max(np.abs(zf - zi)) > 1e-4 or zi.min() < 0 or zi.max() >= Z:
         return None
-    return planes[0, :, :2].astype(np.float32), zi.astype(np.int64)
+    return unit(planes[0, :, :2]).astype(np.float32), zi.astype(np.int64)
 
 
 def estimate_bytes_stacked(shape, n_samples2d, n_planes, n_coils, n_trans, toeplitz=False):

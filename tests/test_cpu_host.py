@@ -251,4 +251,8 @@ def test_stack_layout_detection():
     jitter = t3.copy()
     jitter[60, 0] += 1e-3                                            # plane 2 differs
     assert stack_layout(jitter, (16, 16, 8)) is None
+    near = t3.astype(np.float64)
+    near[60, 0] += 5e-8                                              # within 1e-7: a stack
+    got = stack_layout(near, (16, 16, 8))
+    assert got is not None and np.array_equal(got[1], zi)
     assert stack_layout(np.random.default_rng(1).uniform(-0.5, 0.5, (200, 3)), (8, 8, 8)) is None
