@@ -363,25 +363,49 @@ __global__ __launch_bounds__(kThreads) void k_rows_fwd(RowsArgs a) {
   const bool rok = r < rows;
   const int i0 = rok ? r / a.M1 : 0, i1 = rok ? r - i0 * a.M1 : 0;
   const int per_img = (a.smaps ? a.C : 1) * a.L;  // transforms of one image
-  const float2* src = a.img + (int64_t)(tt / per_img) * N + (int64_t)r * a.M2;
+  const int rl = rok ? r : 0;  // lanes past the last row read row 0 and drop it
+  const float2* src = a.img + (int64_t)(tt / per_img) * N + (int64_t)rl * a.M2;
   const float2* sm =
-      a.smaps ? a.smaps + (int64_t)(a.smaps_all ? tt : (tt / a.L) % a.C) * N + (int64_t)r * a.M2
+      a.smaps ? a.smaps + (int64_t)(a.smaps_all ? tt : (tt / a.L) % a.C) * N + (int64_t)rl * a.M2
               : nullptr;
-  const float2* cm = a.cmap ? a.cmap + (int64_t)(tt % a.L) * N + (int64_t)r * a.M2 : nullptr;
+  const float2* cm = a.cmap ? a.cmap + (int64_t)(tt % a.L) * N + (int64_t)rl * a.M2 : nullptr;
   const float rs = rok ? a.dec0[i0] * a.dec1[i1] * a.scale : 0.f;
   const int M2 = a.M2, h2 = M2 / 2;
 
-  // loads first (overlapping), then the arithmetic and LDS stores
+  // loads first, then the arithmetic and LDS stores
   float2 v[EPT], sv[EPT];
+  if constexpr (FIX > 0) {
+    // all in flight together: no branch and no arithmetic on a loaded value
+    // between them (positions past the row end read its last element and
+    // are dropped).  The lengths fixed at compile time have the registers
+    // for it within the workgroups per CU their LDS allows.
 #pragma unroll
-  for (int j = 0; j < EPT; ++j) {
-    const int x = x0 + j * TPL;
-    v[j] = make_float2(0.f, 0.f);
-    sv[j] = make_float2(1.f, 0.f);
-    if (rok && x < M2) {
-      v[j] = src[x];
-      if (sm) sv[j] = sm[x];
-      if (cm) v[j] = cmul(v[j], cm[x]);
+    for (int j = 0; j < EPT; ++j) v[j] = src[min(x0 + j * TPL, M2 - 1)];
+    if (sm) {
+#pragma unroll
+      for (int j = 0; j < EPT; ++j) sv[j] = sm[min(x0 + j * TPL, M2 - 1)];
+    } else {
+#pragma unroll
+      for (int j = 0; j < EPT; ++j) sv[j] = make_float2(1.f, 0.f);
+    }
+    if (cm) {
+#pragma unroll
+      for (int j = 0; j < EPT; ++j) v[j] = cmul(v[j], cm[min(x0 + j * TPL, M2 - 1)]);
+    }
+#pragma unroll
+    for (int j = 0; j < EPT; ++j)
+      if (!rok) v[j] = make_float2(0.f, 0.f);
+  } else {
+#pragma unroll
+    for (int j = 0; j < EPT; ++j) {
+      const int x = x0 + j * TPL;
+      v[j] = make_float2(0.f, 0.f);
+      sv[j] = make_float2(1.f, 0.f);
+      if (rok && x < M2) {
+        v[j] = src[x];
+        if (sm) sv[j] = sm[x];
+        if (cm) v[j] = cmul(v[j], cm[x]);
+      }
     }
   }
 #pragma unroll
@@ -477,10 +501,20 @@ __global__ __launch_bounds__(kThreads) void k_rows_adj(RowsArgs a) {
       asm volatile("" : "+s"(lg), "+s"(lp));
       fft_lines<S, EPT>(buf, twl, a.ax, lg, lp);
     }
+    const int rl = rok ? r : 0;  // lanes past the last row read row 0's maps and drop them
     const float2* sm =
-        a.smaps ? a.smaps + (int64_t)(a.smaps_all ? tt : (tt / a.L) % a.C) * N + (int64_t)r * a.M2
+        a.smaps ? a.smaps + (int64_t)(a.smaps_all ? tt : (tt / a.L) % a.C) * N + (int64_t)rl * a.M2
                 : nullptr;
-    const float2* cm = a.cmap ? a.cmap + (int64_t)(tt % a.L) * N + (int64_t)r * a.M2 : nullptr;
+    const float2* cm = a.cmap ? a.cmap + (int64_t)(tt % a.L) * N + (int64_t)rl * a.M2 : nullptr;
+    // the coil's map values, all loads in flight together (compile-time
+    // lengths; the others load each where it is used)
+    float2 sv[EPT];
+    if constexpr (FIX > 0) {
+      if (sm) {
+#pragma unroll
+        for (int j = 0; j < EPT; ++j) sv[j] = sm[min(x0 + j * TPL, a.M2 - 1)];
+      }
+    }
 #pragma unroll
     for (int j = 0; j < EPT; ++j) {
       const int x = x0 + j * TPL;
@@ -489,9 +523,8 @@ __global__ __launch_bounds__(kThreads) void k_rows_adj(RowsArgs a) {
         if (rok) {
           v = buf[kept_pos(x, a.M2, n) * LP + l];
           if (sm) {
-            float2 sv = sm[x];
-            sv.y = -sv.y;
-            v = cmul(v, sv);
+            float2 s2 = FIX > 0 ? sv[j] : sm[x];
+            v = cmul(v, make_float2(s2.x, -s2.y));
           }
           if (cm) {
             float2 cv = cm[x];
