@@ -570,9 +570,10 @@ int hipnufft_plan_create(int type, int ndim, const int64_t* nmodes, int iflag, i
   p->spread_group = o.spread_group > 0 ? o.spread_group : 0;  // 0: chosen per launch
   // 2D / 3D: 0 = every transform of a work item in one workgroup (items of
   // up to 128 points in 3D: more, smaller items measured faster at 16 coils
-  // than 256 or 512; 256 in 2D, faster than 128 for radial 256^2 at 8 and
-  // 64 coils and for the stacked 128^3 slices); an explicit group picks the
-  // kernel that stages that many transforms side by side (1024 points)
+  // than 256 or 512; 256 in 2D: the stacked 128^3 slices' type 2 took 2.29
+  // ms against 2.34 at 128, radial 256^2 at 8 and 64 coils the same within
+  // 2%); an explicit group picks the kernel that stages that many
+  // transforms side by side (items of 1024 points)
   p->interp_group = o.interp_group > 0 ? o.interp_group : ndim == 1 ? 4 : 0;
   p->max_sub = o.max_subproblem > 0 ? o.max_subproblem
                : ndim == 3              ? 128
