@@ -717,13 +717,18 @@ __global__ __launch_bounds__(64 * NWAVE, 4 / GH) void k_spread_mfma(SpreadArgs a
     }
   }
   const int ov = kBlkHead + (lane & 15);
+  // the point of the entry this thread stages next, read one round ahead
+  int pnext = q < min(CHUNK, e1 - e0) ? a.list_pts[e0 + q] : 0;
   for (int eb = e0; eb < e1; eb += CHUNK) {
     const int n = min(CHUNK, e1 - eb);
+    const int pcur = pnext;
     if (q < n) {
       BlkPos r;
-      load_blk_pos<D>(a, a.list_pts[eb + q], org, r);
+      load_blk_pos<D>(a, pcur, org, r);
       write_mfma_entry<D, G>(smem + q * PW, r, a, t0, Gv, td, ext, nf, nb, inv_hw, beta);
     }
+    // in flight during this round's lists and products
+    if (q < min(CHUNK, e1 - eb - CHUNK)) pnext = a.list_pts[eb + CHUNK + q];
     __syncthreads();
     int ngroups[BPW];
 #pragma unroll
